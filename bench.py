@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""H-matvec throughput of the MI355X apply on one GPU.
+
+Workload: 2-D kernel matrix, n=16384 points uniform in the unit square
+(seed 0), A_ij = exp(-|x_i - x_j| / 0.2), leaf 64, tolerance 1e-6 (the
+tightest tolerance the reference constructor accepts for this matrix; 1e-8
+makes every sampled block infeasible), full width menu; right-hand sides
+N(0,1) FP64 (seed 1).  One step = one apply y = H X of the resident store.
+
+Copies of the store are rotated (--copies, default 4: ~370 MB > the 256 MB
+last-level cache) so the timed figure is HBM-bound, not cache-resident.
+
+Prints one JSON line: value = algorithmic bytes per apply / time per apply
+(GB/s) for the FP32 split-precision apply at k=1; the FP64 apply and k=32 are
+reported beside it, with relative errors against the FP64 CPU reference.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+T0 = time.time()
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stats: dict, k: int) -> dict:
+    """Bytes an apply must move at minimum, by pass (device formats)."""
+    n = stats["order"]
+    meta = 20 * stats["ncomponents"]  # two FP64 scales + an int32 width
+    right = 4 * stats["right_code_words"] + 4 * stats["ncomponents"] + 8 * n * k + 4 * n
+    left = (4 * stats["left_code_words"] + 16 * stats["ncomponents"] + 8 * stats["dense_values"]
+            + 8 * n * k + 4 * n)
+    total = 4 * stats["code_words"] + meta + 8 * stats["dense_values"] + 16 * n * k + 4 * n
+    return {"total": total, "code_stream": 4 * stats["code_words"], "pass0": right,
+            "pass1": left}
+
+
+def flops(stats: dict, k: int) -> dict:
+    return {"fp_codes": stats["lowrank_flops_per_rhs"] * k,
+            "fp64_dense": stats["dense_flops_per_rhs"] * k}
+
+
+def time_applies(torch, lib, stores, x, y, ws, k, precision, steps, warmup):
+    """Runs warmup + steps applies cycling through `stores`; returns
+    (ms per apply by events around the loop, per-pass ms from library events)."""
+    import ctypes as C
+
+    from qhmat_amd import _native as N
+
+    stream = torch.cuda.current_stream()
+    h = C.c_void_p(stream.cuda_stream)
+    args = (C.c_void_p(x.data_ptr()), x.stride(0), k, C.c_void_p(y.data_ptr()), y.stride(0),
+            precision, h, C.c_void_p(ws.data_ptr()), ws.numel())
+    for i in range(warmup):
+        N.check(lib.qh_apply(stores[i % len(stores)], *args))
+    torch.cuda.synchronize()
+    for s in stores:
+        N.check(lib.qh_profile_begin(s, steps))
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for i in range(steps):
+        N.check(lib.qh_apply(stores[i % len(stores)], *args))
+    end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    passes = [0.0, 0.0]
+    count = 0
+    for s in stores:
+        prof = N.Profile()
+        N.check(lib.qh_profile_end(s, C.byref(prof)))
+        passes[0] += prof.pass_ms[0]
+        passes[1] += prof.pass_ms[1]
+        count += prof.applies
+    return total_ms / steps, [p / max(count, 1) for p in passes]
+
+
+def cpu_baseline(h, x1, budget_s: float):
+    """The reference's compiled CPU matmul (oracle/_ref) when present, else the
+    oracle restatement, on one core, k=1; returns (summary, y)."""
+    from oracle import oracle as O
+
+    tmp = tempfile.mkdtemp(prefix="qhmat-bench-")
+    path = os.path.join(tmp, "store.qhmat")
+    h.save(path)
+    try:
+        if O.ref_available():
+            store, kind = O.RefHMatrix.load(path), "reference"
+        else:
+            store, kind = O.OracleStore.from_file(path, verify_checksum=False), "port"
+    finally:
+        os.remove(path)
+        os.rmdir(tmp)
+    times, y = [], None
+    t_end = time.time() + budget_s
+    while len(times) < 3 or (time.time() < t_end and len(times) < 50):
+        t = time.perf_counter()
+        y = store.matmul(x1)
+        times.append(time.perf_counter() - t)
+    return {"kind": kind, "seconds": statistics.median(times), "runs": len(times)}, y
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--copies", type=int, default=4)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import qhmat_amd as q
+    from qhmat_amd import _native as N
+    from qhmat_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    torch.cuda.set_device(0)
+    log(f"building square_exp n={args.n}")
+    w = W.square_exp(args.n)
+    h = w.build()
+    native = h.native
+    stores = [native.device_store(0)] + [native.upload_copy(0) for _ in range(args.copies - 1)]
+    stats = native.device_stats(0)
+    log(f"store: {stats['nblocks']} blocks, {stats['codes']} codes, "
+        f"{4 * stats['code_words'] / 1e6:.1f} MB code stream, {args.copies} copies resident")
+    lib = N.lib()
+    results = {}
+    for k in (1, 32):
+        x_np = w.rhs_matrix(k)
+        x = torch.from_numpy(x_np).cuda()
+        y = torch.empty_like(x)
+        from qhmat_amd import device as D
+
+        ws = torch.empty(max(D.workspace_bytes(native, k), 8), dtype=torch.uint8, device="cuda")
+        for prec_name, prec in (("fp32split", N.PRECISION_FP32SPLIT), ("fp64", N.PRECISION_FP64)):
+            ms, passes = time_applies(torch, lib, stores, x, y, ws, k, prec, args.steps,
+                                      args.warmup)
+            out = native.matmul(x_np, True, precision=prec_name)
+            results[(k, prec_name)] = {"ms": ms, "passes_ms": passes, "y": out}
+            log(f"k={k} {prec_name}: {ms * 1e3:.1f} us/apply "
+                f"(passes {passes[0] * 1e3:.1f} + {passes[1] * 1e3:.1f} us)")
+
+    base = None
+    y_cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline (1 core, k=1)")
+        base, y_cpu = cpu_baseline(h, w.rhs_matrix(1), args.cpu_seconds)
+        log(f"cpu baseline ({base['kind']}): {base['seconds'] * 1e3:.1f} ms/apply")
+
+    def summary(k, prec):
+        r = results[(k, prec)]
+        b = algorithmic_bytes(stats, k)
+        f = flops(stats, k)
+        t = r["ms"] * 1e-3
+        out = {
+            "ms_per_apply": r["ms"],
+            "gbps": b["total"] / t / 1e9,
+            "code_stream_gbps": b["code_stream"] / t / 1e9,
+            "matvecs_per_s": k / t,
+            "pass_ms": r["passes_ms"],
+            "t_roof_ms": max(b["total"] / (HBM_PEAK_GBS * 1e9),
+                             f["fp_codes"] / ((FP32_PEAK_TFLOPS if prec == "fp32split"
+                                               else FP64_PEAK_TFLOPS) * 1e12)
+                             + f["fp64_dense"] / (FP64_PEAK_TFLOPS * 1e12)) * 1e3,
+        }
+        out["roofline_frac"] = out["t_roof_ms"] / r["ms"]
+        if k == 1 and y_cpu is not None:
+            yy = r["y"].reshape(-1)
+            out["rel_err_vs_cpu_fp64"] = float(np.linalg.norm(yy - y_cpu.reshape(-1))
+                                               / np.linalg.norm(y_cpu))
+        return out
+
+    head = summary(1, "fp32split")
+    b1 = algorithmic_bytes(stats, 1)
+    r1 = results[(1, "fp32split")]
+    dominant = int(np.argmax(r1["passes_ms"]))
+    dom_ms = r1["passes_ms"][dominant]
+    dom_bytes = b1[f"pass{dominant}"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    line = {
+        "metric": "H-matvec GB/s (% HBM peak) FP32 vs FP64 at rel err; served p50/p99 ms",
+        "value": round(head["gbps"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r1["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (codes x vector, split for 32-bit codes), fp64 scales/diagonal/accumulation",
+        "data": "synthetic (seeded points and N(0,1) right-hand sides)",
+        "config": {"workload": "square_exp: n=16384 uniform 2-D, exp(-r/0.2), leaf 64, "
+                               "tol 1e-6, widths 4..32, k=1, 4 rotated store copies",
+                   "n": args.n, "rhs": 1, "copies": args.copies,
+                   "store_code_stream_bytes": b1["code_stream"],
+                   "algorithmic_bytes_per_apply": b1["total"]},
+        "pct_hbm_peak": round(100 * head["gbps"] / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": ["right_factor_pass", "row_tile_pass"][dominant],
+                     "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
+        "fp32split_k1": head,
+        "fp64_k1": summary(1, "fp64"),
+        "fp32split_k32": summary(32, "fp32split"),
+        "fp64_k32": summary(32, "fp64"),
+    }
+    if base is not None:
+        line["cpu_baseline"] = {
+            "value": round(b1["total"] / base["seconds"] / 1e9, 4), "unit": "GB/s",
+            "cores": 1, "kind": base["kind"],
+            "sample": f"{base['runs']} applies of the same store at k=1 (median "
+                      f"{base['seconds'] * 1e3:.1f} ms/apply)"}
+    for key in ("fp32split_k1", "fp64_k1", "fp32split_k32", "fp64_k32"):
+        line[key] = {a: (round(v, 6) if isinstance(v, float) else v)
+                     for a, v in line[key].items()}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,219 @@
+/*
+ * qhmat_hip.h -- C ABI of the MI355X (gfx950) H-matrix apply library
+ * (qhmat_amd/libqhmat_hip.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. The device path -- the drop-in for the reference's GPU backend.  The
+ *     reference binding NativeHMatrix.matmul(x, use_cuda=True)
+ *     (src/qhmat/bindings.cpp:152-179) calls HMatrix::matmul_cuda
+ *     (src/qhmat/cuda.cu:106-164), which flattens and uploads the whole store
+ *     on every call.  Here the store is uploaded once (qh_store_create) and
+ *     every apply (qh_apply) works on device pointers on a caller stream.
+ *
+ *  2. The host store -- what NativeHMatrix holds (src/qhmat/core.hpp:52-102,
+ *     bindings.cpp:124-198): block insertion, packing, finalize, the host
+ *     apply, reconstruction, accounting and QHMAT001 files.  It is the
+ *     source the device store is uploaded from.
+ *
+ * All functions return a qh_status; on failure qh_last_error() returns the
+ * message for the calling thread.  Status codes map onto the exceptions the
+ * reference raises: QH_EINVAL -> std::invalid_argument (Python ValueError),
+ * QH_ERUNTIME / QH_ENODEV -> std::runtime_error (RuntimeError),
+ * QH_EOVERFLOW -> std::overflow_error (OverflowError), QH_ENOMEM -> MemoryError.
+ *
+ * No symbol here takes a torch type: pointers, sizes and a hipStream_t passed
+ * as void*.
+ */
+#ifndef QHMAT_HIP_H
+#define QHMAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    QH_OK = 0,
+    QH_EINVAL = 1,
+    QH_ERUNTIME = 2,
+    QH_EOVERFLOW = 3,
+    QH_ENOMEM = 4,
+    QH_ENODEV = 5
+} qh_status;
+
+/* Arithmetic of the off-diagonal code x vector products. */
+typedef enum {
+    /* FP64 throughout: the parity anchor against the FP64 CPU apply. */
+    QH_PRECISION_FP64 = 0,
+    /* Codes of <= 24 bits as FP32 (exact); wider codes split into a signed
+     * high half and an unsigned 16-bit low half, each exact in FP32 and
+     * accumulated apart.  FP32 accumulation inside a block; scales, the
+     * diagonal blocks and the cross-block sum into y in FP64. */
+    QH_PRECISION_FP32SPLIT = 1
+} qh_precision;
+
+typedef struct qh_host qh_host;   /* host packed store (NativeHMatrix)        */
+typedef struct qh_store qh_store; /* device-resident store, uploaded once     */
+
+/* Flat, read-only view of a finalized host store in the reference's packed
+ * format.  This is the shape any caller holding a reference HMatrix can
+ * produce from its private members (core.hpp:77-95): per block rectangle,
+ * dense values or widths + scales + the left and right word streams, where
+ * component j of a factor with `rows` rows occupies ceil(rows*b_j/32) words
+ * starting right after component j-1 (io.cpp:233-243). */
+typedef struct {
+    uint64_t order;
+    const uint64_t *perm;        /* [order] clustered position -> source index */
+    uint64_t nblocks;
+    const uint64_t *rect;        /* [4*nblocks] row0,row1,col0,col1            */
+    const uint8_t *is_dense;     /* [nblocks]                                  */
+    const uint64_t *rank;        /* [nblocks] retained components (0 if dense) */
+    const double *const *dense;  /* [nblocks] row-major m*m values, dense only */
+    const int32_t *const *widths;      /* [nblocks] -> rank widths            */
+    const double *const *lscale;       /* [nblocks] -> rank left scales       */
+    const double *const *rscale;       /* [nblocks] -> rank right scales      */
+    const uint32_t *const *lwords;     /* [nblocks] -> left factor words      */
+    const uint32_t *const *rwords;     /* [nblocks] -> right factor words     */
+} qh_store_desc;
+
+/* Resident sizes and per-apply algorithmic counts of a device store. */
+typedef struct {
+    uint64_t order, nblocks, ndense, nlowrank, ncomponents;
+    uint64_t code_words;          /* packed code stream, 32-bit words         */
+    uint64_t left_code_words, right_code_words;
+    uint64_t codes;               /* sum over low-rank blocks of (m+n)*r      */
+    uint64_t dense_values;        /* FP64 values in diagonal blocks           */
+    uint64_t device_bytes;        /* everything resident on the device        */
+    uint64_t lowrank_flops_per_rhs; /* sum 2 r (m+n)                          */
+    uint64_t dense_flops_per_rhs;   /* sum 2 m n over dense blocks            */
+    uint64_t tiles, items, entries;
+    int32_t device;
+} qh_store_stats;
+
+const char *qh_last_error(void);
+/* Version of this library ("major.minor.patch"). */
+const char *qh_version(void);
+
+/* ---- device path ------------------------------------------------------- */
+
+/* Number of visible HIP devices (0 when none). */
+qh_status qh_device_count(int *count);
+
+/* Uploads a finalized store to `device` once.  Replaces the per-call flatten +
+ * upload at cuda.cu:119-150. */
+qh_status qh_store_create(const qh_store_desc *desc, int device, qh_store **out);
+qh_status qh_store_destroy(qh_store *store);
+qh_status qh_store_get_stats(const qh_store *store, qh_store_stats *out);
+
+/* Workspace bytes qh_apply needs for `k` right-hand sides. */
+qh_status qh_apply_workspace_bytes(const qh_store *store, int64_t k, uint64_t *bytes);
+
+/* y = H x on the device (cuda.cu:106-164 semantics, minus the copies).
+ * x: order x k FP64, row-major, leading dimension ldx >= k, source order.
+ * y: order x k FP64, leading dimension ldy >= k; every entry is written.
+ * stream: a hipStream_t (NULL = null stream); the call is stream-ordered and
+ * does not synchronise.  workspace: device memory of at least
+ * qh_apply_workspace_bytes, or NULL to let the library take it from the
+ * stream-ordered allocator.  x must be finite (callers check; the numpy path
+ * of the Python wrapper raises the reference's "nonfinite right-hand side").
+ * Concurrent calls on one store are safe when each has its own workspace. */
+qh_status qh_apply(const qh_store *store, const double *x, int64_t ldx, int64_t k, double *y,
+                   int64_t ldy, int precision, void *stream, void *workspace,
+                   uint64_t workspace_bytes);
+
+/* Per-pass timing of qh_apply on the stream the passes are launched on.
+ * qh_profile_begin arms hipEvents around each of the two passes of the next
+ * `max_applies` calls on this store (the events are created here, not in the
+ * timed calls); qh_profile_end waits for the recorded events, returns the
+ * totals and disarms.  Pass 0 = right-factor pass, pass 1 = row-tile pass. */
+typedef struct {
+    uint64_t applies;
+    double pass_ms[2];
+} qh_profile;
+qh_status qh_profile_begin(qh_store *store, uint64_t max_applies);
+qh_status qh_profile_end(qh_store *store, qh_profile *out);
+
+/* Test hook: the integer codes of one side (0 = left, 1 = right) of block
+ * `block` (insertion index), decoded on the device from the resident words
+ * into out[j*rows + i] (int32, device memory, rank*rows entries).  They must
+ * equal the host unpacker bit for bit (core.cpp:269-280). */
+qh_status qh_decode_codes(const qh_store *store, uint64_t block, int side, int32_t *out,
+                          void *stream);
+
+/* ---- host store (NativeHMatrix, bindings.cpp:124-198) ------------------- */
+
+qh_status qh_host_create(uint64_t order, const uint64_t *perm, qh_host **out);
+qh_status qh_host_destroy(qh_host *h);
+/* core.cpp:224-233; values row-major (row1-row0) x (col1-col0). */
+qh_status qh_host_add_dense(qh_host *h, uint64_t row0, uint64_t row1, uint64_t col0,
+                            uint64_t col1, int admissible, const double *values,
+                            uint64_t count);
+/* core.cpp:284-330; u: m x rank and v: n x rank column-major, sigma: rank. */
+qh_status qh_host_add_low_rank(qh_host *h, uint64_t row0, uint64_t row1, uint64_t col0,
+                               uint64_t col1, int admissible, const double *u,
+                               const double *sigma, uint64_t rank, const double *v,
+                               const int32_t *widths, uint64_t width_count,
+                               uint64_t menu_size, double truncation_error);
+qh_status qh_host_finalize(qh_host *h);
+/* core.cpp:360-390 semantics on the host (device="cpu"): x, y order x rhs. */
+qh_status qh_host_matmul(const qh_host *h, const double *x, uint64_t rhs, double *y,
+                         int threads);
+/* core.cpp:392-410: order*order values, source order. */
+qh_status qh_host_reconstruct(const qh_host *h, double *out);
+
+typedef struct {
+    uint64_t order, nblocks;
+    int32_t finalized;
+    uint64_t logical_bits, allocated_bytes, resident_bytes;
+    double reconstruction_bound;
+} qh_host_info;
+qh_status qh_host_get_info(const qh_host *h, qh_host_info *out);
+
+typedef struct {
+    uint64_t row0, row1, col0, col1;
+    int32_t admissible, dense;
+    uint64_t rank;
+    double truncation_error, quantization_bound;
+    uint64_t logical_bits, allocated_bytes;
+} qh_block_info;
+/* core.cpp:412-436 for block `index` (insertion order). */
+qh_status qh_host_block_info(const qh_host *h, uint64_t index, qh_block_info *out);
+/* Clustered -> source permutation, order entries. */
+qh_status qh_host_permutation(const qh_host *h, uint64_t *out);
+/* Words / scales / widths of one block side (0 left, 1 right): sizes first
+ * (pass NULL buffers), then the copy. */
+qh_status qh_host_block_words(const qh_host *h, uint64_t index, int side, uint32_t *words,
+                              uint64_t *word_count, double *scales, int32_t *widths);
+/* io.cpp:127-257 format.  save refuses to replace an existing file. */
+qh_status qh_host_save(const qh_host *h, const char *path);
+qh_status qh_host_load(const char *path, qh_host **out);
+/* Fills `desc` with pointers into `h` (valid while h lives, unmodified).
+ * The pointer tables are owned by h. */
+qh_status qh_host_describe(qh_host *h, qh_store_desc *desc);
+
+/* ---- construction helpers (core.cpp:56-203) ----------------------------- */
+
+qh_status qh_coefficient(int bits, double *out);
+/* core.cpp:61-115.  bits_out: rank entries. */
+qh_status qh_allocate(const double *sigma, uint64_t rank, const int32_t *menu,
+                      uint64_t menu_count, double budget, uint64_t rows, uint64_t cols,
+                      int32_t *bits_out, double *bound, uint64_t *payload_bits,
+                      int32_t *feasible);
+
+typedef struct qh_partition qh_partition;
+/* core.cpp:117-203; points/lower/upper: count x dim row-major. */
+qh_status qh_partition_create(const double *points, const double *lower, const double *upper,
+                              uint64_t count, uint64_t dim, uint64_t leaf_size, double eta,
+                              qh_partition **out);
+qh_status qh_partition_sizes(const qh_partition *p, uint64_t *count, uint64_t *nblocks);
+/* perm: count entries; rects: 5 per block (row0,row1,col0,col1,admissible). */
+qh_status qh_partition_copy(const qh_partition *p, uint64_t *perm, uint64_t *rects);
+qh_status qh_partition_destroy(qh_partition *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QHMAT_HIP_H */

@@ -1,0 +1,139 @@
+"""Device-side apply on PyTorch-ROCm tensors.
+
+PyTorch supplies device memory and the current HIP stream; the arithmetic is
+the library's HIP kernels (qh_apply).  Nothing here computes on the host.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+
+_PRECISIONS = {
+    "fp64": _native.PRECISION_FP64,
+    "fp32": _native.PRECISION_FP32SPLIT,
+    "fp32split": _native.PRECISION_FP32SPLIT,
+}
+
+
+def precision_code(precision: str) -> int:
+    try:
+        return _PRECISIONS[precision]
+    except KeyError:
+        raise ValueError("precision must be 'fp64' or 'fp32split'") from None
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def require_device() -> None:
+    """Raises the reference's error when no GPU is visible (cuda.cu:115-117)."""
+    torch = _torch()
+    if not torch.cuda.is_available() or _native.device_count() == 0:
+        raise RuntimeError("no CUDA device is available")
+
+
+def workspace_bytes(native, k: int, device: int = 0) -> int:
+    out = C.c_uint64()
+    _native.check(_native.lib().qh_apply_workspace_bytes(native.device_store(device), int(k),
+                                                         C.byref(out)))
+    return int(out.value)
+
+
+def apply(native, x, y=None, *, precision: str = "fp64", stream=None, workspace=None):
+    """y = H x for an FP64 device tensor x of shape (n,) or (n, k).
+
+    Rows of x and y must be contiguous (stride 1 along k); any row stride is
+    accepted.  The call is ordered on `stream` (default: the current stream of
+    x's device) and does not synchronise.  `workspace`: optional uint8 device
+    tensor of at least workspace_bytes(native, k).
+    """
+    torch = _torch()
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise TypeError("x must be a device tensor")
+    if x.dtype != torch.float64:
+        raise TypeError("x must be float64")
+    vec = x.dim() == 1
+    x2 = x.unsqueeze(1) if vec else x
+    if x2.dim() != 2:
+        raise ValueError("right-hand side must be one- or two-dimensional")
+    n = native.order
+    if x2.shape[0] != n:
+        raise ValueError("right-hand side has the wrong row count")
+    k = x2.shape[1]
+    if k == 0:
+        raise ValueError("right-hand-side shape mismatch")
+    if x2.stride(1) != 1 and k > 1:
+        x2 = x2.contiguous()
+    device = x2.device.index if x2.device.index is not None else torch.cuda.current_device()
+    if y is None:
+        y2 = torch.empty((n, k), dtype=torch.float64, device=x2.device)
+    else:
+        y2 = y.unsqueeze(1) if y.dim() == 1 else y
+        if y2.shape != (n, k) or y2.dtype != torch.float64 or y2.device != x2.device:
+            raise ValueError("result tensor has the wrong shape, dtype or device")
+        if y2.stride(1) != 1 and k > 1:
+            raise ValueError("result rows must be contiguous")
+    store = native.device_store(device)
+    if stream is None:
+        stream = torch.cuda.current_stream(x2.device)
+    handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+    ws_ptr, ws_bytes = None, 0
+    if workspace is None:
+        need = workspace_bytes(native, k, device)
+        if need:
+            with torch.cuda.stream(stream) if hasattr(stream, "cuda_stream") else _null():
+                workspace = torch.empty(need, dtype=torch.uint8, device=x2.device)
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    ldx = x2.stride(0) if k > 1 else max(x2.stride(0), 1)
+    ldy = y2.stride(0) if k > 1 else max(y2.stride(0), 1)
+    _native.check(_native.lib().qh_apply(
+        store, C.c_void_p(x2.data_ptr()), ldx, k, C.c_void_p(y2.data_ptr()), ldy,
+        precision_code(precision), C.c_void_p(handle), C.c_void_p(ws_ptr), ws_bytes))
+    if y is not None:
+        return y
+    return y2.squeeze(1) if vec else y2
+
+
+class _null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+def apply_numpy(native, a: np.ndarray, precision: str = "fp64") -> np.ndarray:
+    """numpy (n, k) in, numpy (n, k) out, through the device (blocking)."""
+    require_device()
+    torch = _torch()
+    x = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    y = apply(native, x, precision=precision)
+    torch.cuda.current_stream().synchronize()
+    return y.cpu().numpy()
+
+
+def decode_codes(native, block: int, side: int) -> np.ndarray:
+    """Codes of one block side, (rank, rows) int32, decoded by the device."""
+    require_device()
+    torch = _torch()
+    info = native.blocks[block]
+    if info.dense:
+        raise ValueError("dense blocks carry no codes")
+    r = info.rectangle
+    rows = (r.row1 - r.row0) if side == 0 else (r.col1 - r.col0)
+    out = torch.zeros((max(info.rank, 1), rows), dtype=torch.int32, device="cuda")
+    store = native.device_store(torch.cuda.current_device())
+    stream = torch.cuda.current_stream()
+    _native.check(_native.lib().qh_decode_codes(store, int(block), int(side),
+                                                C.c_void_p(out.data_ptr()),
+                                                C.c_void_p(stream.cuda_stream)))
+    stream.synchronize()
+    return out.cpu().numpy()[: info.rank]
