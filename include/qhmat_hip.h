@@ -136,6 +136,12 @@ typedef struct {
 qh_status qh_profile_begin(qh_store *store, uint64_t max_applies);
 qh_status qh_profile_end(qh_store *store, qh_profile *out);
 
+/* Layout check without a device: builds the device plan of `desc` on the
+ * host and runs both passes' index arithmetic against its arrays with every
+ * access bounds-checked, writing y = H x (FP64, order x k row-major).  A
+ * test hook for the tile-major layout; no apply path calls it. */
+qh_status qh_plan_emulate(const qh_store_desc *desc, const double *x, int64_t k, double *y);
+
 /* Test hook: the integer codes of one side (0 = left, 1 = right) of block
  * `block` (insertion index), decoded on the device from the resident words
  * into out[j*rows + i] (int32, device memory, rank*rows entries).  They must

@@ -104,6 +104,7 @@ _sig("qh_store_get_stats", _p, C.POINTER(StoreStats))
 _sig("qh_apply_workspace_bytes", _p, _i64, C.POINTER(_u64))
 _sig("qh_apply", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p, _p, _u64)
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
+_sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)
 _sig("qh_profile_begin", _p, _u64)
 _sig("qh_profile_end", _p, C.POINTER(Profile))
 _sig("qh_host_create", _u64, _p, C.POINTER(_p))
@@ -457,6 +458,16 @@ class NativeHMatrix:
         handle = _p()
         check(_lib.qh_store_create(C.byref(desc), int(device), C.byref(handle)))
         return handle
+
+    def plan_emulate(self, x) -> np.ndarray:
+        """Host check of the device layout (qh_plan_emulate): y = H x computed
+        by replaying the kernels' index arithmetic with bounds checks."""
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(self.order, -1))
+        y = np.zeros_like(a)
+        desc = StoreDesc()
+        check(_lib.qh_host_describe(self._h, C.byref(desc)))
+        check(_lib.qh_plan_emulate(C.byref(desc), _ptr(a), a.shape[1], _ptr(y)))
+        return y
 
     def device_stats(self, device: int = 0) -> dict:
         stats = StoreStats()

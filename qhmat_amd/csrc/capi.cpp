@@ -131,6 +131,17 @@ qh_status qh_apply(const qh_store* store, const double* x, int64_t ldx, int64_t 
     });
 }
 
+qh_status qh_plan_emulate(const qh_store_desc* desc, const double* x, int64_t k, double* y) {
+    return guarded([&] {
+        need(desc, "description");
+        need(x, "right-hand side");
+        need(y, "result");
+        if (k <= 0) throw std::invalid_argument("right-hand-side shape mismatch");
+        const qhamd::Plan plan = qhamd::build_plan(*desc);
+        qhamd::emulate_plan(plan, x, k, y);
+    });
+}
+
 qh_status qh_profile_begin(qh_store* store, uint64_t max_applies) {
     return guarded([&] {
         need(store, "store");

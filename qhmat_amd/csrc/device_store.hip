@@ -1,20 +1,30 @@
 // MI355X (gfx950) apply of a qhmat H-matrix, y = H x.
 //
-// The packed store is uploaded once.  An apply is two launches on the caller's
-// stream, both deterministic (fixed reduction order, one writer per output):
+// The packed store is uploaded once in the tile-major layout described in
+// device_store.hpp.  An apply is two launches on the caller's stream, both
+// deterministic (fixed summation order, one writer per output element, no
+// floating-point atomics):
 //
-//  1. right-factor pass (one wave per work item = <=256 columns of one
-//     low-rank block): lanes own columns, decode their b-bit codes straight
-//     from the resident words, multiply by x gathered through the permutation
-//     and reduce across the wave per component:
-//         zp[item][j][p] = sum_{c in item} codeR(c, j) * x[perm[col0 + c], p]
-//  2. row-tile pass (one workgroup of four waves per tile of <=64 rows of one
-//     row slab): lanes own rows; the waves split the tile's blocks and
-//     accumulate
-//         y[row, p] += sum_j codeL(row, j) * (sL_j sR_j sum_items zp[.][j][p])
-//     for low-rank blocks and D[row, :] x for the diagonal block, then the
-//     four partial rows are added in wave order and written once to
-//     y[perm[row], p].
+//  1. right-factor pass: one wave per (low-rank block, column tile).  The
+//     wave streams the tile's chunk into LDS with 16-byte loads and the
+//     tile's x rows (gathered through the permutation) next to it; four
+//     lanes share each component, decode their codes from LDS
+//     (v_alignbit_b32 + v_bfe_i32), multiply by x and are reduced in the
+//     quad.  Output: partial dot products
+//         zp[zbase + s*rank + j][p] = sum_{c in tile s} codeR(c, j) x[c, p].
+//  2. row-tile pass: one workgroup of eight waves per row tile.  The waves
+//     split the tile's blocks in insertion order; a wave streams each
+//     block's chunk for the tile into LDS and every lane (one row) decodes
+//     its code of each component and accumulates
+//         y[row, p] += sum_j codeL(row, j) * (sL_j sR_j sum_s zp[.][p]),
+//     plus D[row, :] x for the diagonal block.  The eight partial rows are
+//     added in wave order and written once to y[perm[row], p].
+//
+// FP64 mode computes in FP64 throughout.  FP32 split mode converts codes of
+// at most 24 bits and x to FP32 (exact for the codes), splits wider codes
+// into a signed high half and an unsigned 16-bit low half accumulated apart,
+// sums each block's contribution to a row in FP32 and everything across
+// blocks, the scales and the diagonal in FP64.
 //
 // Reference: src/qhmat/cuda.cu:64-164 (one thread per output entry scanning
 // every block, store uploaded per call) and core.cpp:360-390 (FP64 loop nest).
@@ -51,149 +61,173 @@ struct DeviceScope {
     }
 };
 
+void require(bool ok, const char* message) {
+    if (!ok) throw std::invalid_argument(message);
+}
+
 // ---------------------------------------------------------------------------
 // Device helpers
 
-// Sign-extended b-bit code number `idx` of the component stream at `cw`.
-// The word pair is always read: a code that does not straddle a word boundary
-// takes all its bits from the low word, and the array is padded so the
-// second read stays in bounds.  v_alignbit_b32 + v_bfe_i32.
-__device__ __forceinline__ int decode_code(const std::uint32_t* __restrict__ cw, std::uint32_t idx,
+// Code number `idx` of the stream at `seg` (LDS or global).  The word pair is
+// always read: a code that does not straddle takes all its bits from the low
+// word, and one word past every staged stream is kept readable.
+__device__ __forceinline__ int decode_code(const std::uint32_t* seg, std::uint32_t idx,
                                            std::uint32_t bits) {
     const std::uint32_t pos = idx * bits;
     const std::uint32_t w = pos >> 5;
-    const std::uint32_t lo = cw[w];
-    const std::uint32_t hi = cw[w + 1];
-    const std::uint32_t raw = __builtin_amdgcn_alignbit(hi, lo, pos & 31u);
+    const std::uint32_t raw = __builtin_amdgcn_alignbit(seg[w + 1], seg[w], pos & 31u);
     return bits == 32 ? static_cast<int>(raw) : __builtin_amdgcn_sbfe(raw, 0u, bits);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+__host__ __device__ __forceinline__ std::uint32_t stream_words(std::uint32_t count,
+                                                               std::uint32_t bits) {
+    return (count * bits + 31u) >> 5;
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// LDS written by some lanes of a wave and read by others: keep the compiler
+// from moving accesses across this point and wait for the LDS queue.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ std::uint64_t component_words(std::uint32_t rows, std::uint32_t bits) {
-    return (std::uint64_t(rows) * bits + 31u) >> 5;
+// Copies `quads` 16-byte groups from global (16-byte aligned) to LDS.
+__device__ __forceinline__ void stage_quads(std::uint32_t* __restrict__ lds,
+                                            const std::uint32_t* __restrict__ src,
+                                            std::uint32_t quads, int lane) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(lds);
+    for (std::uint32_t q = lane; q < quads; q += 64) d[q] = s[q];
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1: right factor.  KB right-hand sides per launch row (grid.y).
+// Pass 1: right factor, one wave per (block, column tile).
 template <int KB, bool FP32>
 __global__ __launch_bounds__(64) void right_factor_pass(
-    const ItemDev* __restrict__ items, const LowRankDev* __restrict__ blocks,
+    const UnitDev* __restrict__ units, const BlockDev* __restrict__ blocks,
     const std::int32_t* __restrict__ widths, const std::uint32_t* __restrict__ words,
     const std::int32_t* __restrict__ perm, const double* __restrict__ x, std::int64_t ldx, int k,
     double* __restrict__ zp) {
-    const ItemDev it = items[blockIdx.x];
-    const LowRankDev B = blocks[it.block];
+    using Real = typename std::conditional<FP32, float, double>::type;
+    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
+    __shared__ Real xs[kTile][KB];
+
+    const UnitDev u = units[blockIdx.x];
+    const BlockDev B = blocks[u.block];
     const int lane = threadIdx.x;
     const int p0 = blockIdx.y * KB;
     const int kk = min(KB, k - p0);
-    constexpr int kSteps = kItemCols / 64;
-    using Real = typename std::conditional<FP32, float, double>::type;
+    const std::uint32_t C = u.cols;
 
-    // x rows of this lane's columns, gathered through the permutation once
-    // (kept in registers for small KB, re-read through the cache otherwise).
-    constexpr bool kKeep = KB <= 4;
-    std::int64_t src[kSteps];
-    Real xv[kKeep ? kSteps : 1][kKeep ? KB : 1];
+    if (lane < int(C)) {
+        const double* xr = x + std::int64_t(perm[u.col0 + lane]) * ldx + p0;
 #pragma unroll
-    for (int t = 0; t < kSteps; ++t) {
-        const std::uint32_t c = it.c0 + t * 64 + lane;
-        src[t] = c < it.c1 ? std::int64_t(perm[B.col0 + c]) * ldx + p0 : 0;
-        if constexpr (kKeep) {
-#pragma unroll
-            for (int p = 0; p < KB; ++p)
-                xv[t][p] = (c < it.c1 && p < kk) ? Real(x[src[t] + p]) : Real(0);
-        }
+        for (int p = 0; p < KB; ++p) xs[lane][p] = p < kk ? Real(xr[p]) : Real(0);
     }
-    auto xat = [&](int t, int p) -> Real {
-        if constexpr (kKeep) return xv[t][p];
-        else return p < kk ? Real(x[src[t] + p]) : Real(0);
-    };
 
-    std::uint64_t woff = B.rbase;
-    for (std::uint32_t j = 0; j < B.rank; ++j) {
-        const std::uint32_t bits = widths[B.comp_base + j];
-        const std::uint32_t* cw = words + woff;
-        double total[KB];
+    const int g = lane >> 2;  // component slot of this lane in a pass
+    const int q = lane & 3;   // columns q, q+4, q+8, ... of the tile
+    std::uint32_t done = 0;   // chunk words of the components already processed
+    for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
+        const std::uint32_t jn = min(std::uint32_t(kPassComponents), B.rank - j0);
+        // widths and stream offsets of this pass's components (prefix scan)
+        const std::uint32_t wl = lane < int(jn) ? std::uint32_t(widths[B.comp_base + j0 + lane]) : 0u;
+        const std::uint32_t nwl = stream_words(C, wl);
+        std::uint32_t incl = nwl;
+#pragma unroll
+        for (int off = 1; off < kPassComponents; off <<= 1) {
+            const std::uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        const std::uint32_t total = __shfl(incl, kPassComponents - 1, 64);
+        const std::uint32_t bits = __shfl(wl, g & (kPassComponents - 1), 64);
+        const std::uint32_t seg = __shfl(incl - nwl, g & (kPassComponents - 1), 64);
+
+        const std::uint64_t first = u.chunk + done;
+        const std::uint32_t delta = std::uint32_t(first & 3u);
+        stage_quads(stage, words + (first - delta), (total + delta + 1 + 3) >> 2, lane);
+        wave_lds_sync();
+
+        double tot[KB];
+        const bool mine = g < int(jn);
+        const std::uint32_t* sw = stage + delta + seg;
         if (!FP32 || bits <= 24) {
             Real part[KB];
 #pragma unroll
             for (int p = 0; p < KB; ++p) part[p] = Real(0);
+            if (mine) {
+                for (std::uint32_t c = q; c < C; c += 4) {
+                    const Real code = Real(decode_code(sw, c, bits));
 #pragma unroll
-            for (int t = 0; t < kSteps; ++t) {
-                const std::uint32_t c = it.c0 + t * 64 + lane;
-                if (c < it.c1) {
-                    const Real code = Real(decode_code(cw, c, bits));
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) part[p] = fma(code, xat(t, p), part[p]);
+                    for (int p = 0; p < KB; ++p) part[p] = fma(code, xs[c][p], part[p]);
                 }
             }
 #pragma unroll
-            for (int p = 0; p < KB; ++p) total[p] = double(wave_sum(part[p]));
+            for (int p = 0; p < KB; ++p) {
+                Real v = part[p];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                tot[p] = double(v);
+            }
         } else {
             // 25..32-bit codes: signed high half and unsigned low 16 bits,
             // both exact in FP32, summed apart and joined in FP64.
             float hi[KB], lo[KB];
 #pragma unroll
             for (int p = 0; p < KB; ++p) hi[p] = lo[p] = 0.0f;
+            for (std::uint32_t c = q; c < C; c += 4) {
+                const int code = decode_code(sw, c, bits);
+                const float h = float(code >> 16), l = float(code & 0xFFFF);
 #pragma unroll
-            for (int t = 0; t < kSteps; ++t) {
-                const std::uint32_t c = it.c0 + t * 64 + lane;
-                if (c < it.c1) {
-                    const int code = decode_code(cw, c, bits);
-                    const float h = float(code >> 16);
-                    const float l = float(code & 0xFFFF);
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) {
-                        const float xf = float(xat(t, p));
-                        hi[p] = fmaf(h, xf, hi[p]);
-                        lo[p] = fmaf(l, xf, lo[p]);
-                    }
+                for (int p = 0; p < KB; ++p) {
+                    hi[p] = fmaf(h, float(xs[c][p]), hi[p]);
+                    lo[p] = fmaf(l, float(xs[c][p]), lo[p]);
                 }
             }
 #pragma unroll
-            for (int p = 0; p < KB; ++p)
-                total[p] = double(wave_sum(hi[p])) * 65536.0 + double(wave_sum(lo[p]));
+            for (int p = 0; p < KB; ++p) {
+                float h = hi[p], l = lo[p];
+                h += __shfl_xor(h, 1, 64);
+                h += __shfl_xor(h, 2, 64);
+                l += __shfl_xor(l, 1, 64);
+                l += __shfl_xor(l, 2, 64);
+                tot[p] = double(h) * 65536.0 + double(l);
+            }
         }
-        if (lane == 0) {
-            double* out = zp + (std::uint64_t(it.zbase) + j) * std::uint64_t(k) + p0;
+        if (mine && q == 0) {
+            double* out = zp + (std::uint64_t(u.zrow) + j0 + g) * std::uint64_t(k) + p0;
 #pragma unroll
             for (int p = 0; p < KB; ++p)
-                if (p < kk) out[p] = total[p];
+                if (p < kk) out[p] = tot[p];
         }
-        woff += component_words(B.n, bits);
+        done += total;
+        wave_lds_sync();  // the next pass restages
     }
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: row tiles.
+// Pass 2: row tiles, kRowWaves waves per tile.
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kWaves) void row_tile_pass(
+__global__ __launch_bounds__(64 * kRowWaves) void row_tile_pass(
     const TileDev* __restrict__ tiles, const EntryDev* __restrict__ entries,
-    const LowRankDev* __restrict__ blocks, const DenseDev* __restrict__ denses,
-    const ItemDev* __restrict__ items, const std::int32_t* __restrict__ widths,
-    const double* __restrict__ lscale, const double* __restrict__ rscale,
-    const std::uint32_t* __restrict__ words, const double* __restrict__ dvals,
-    const std::int32_t* __restrict__ perm, const double* __restrict__ zp,
-    const double* __restrict__ x, std::int64_t ldx, int k, double* __restrict__ y,
-    std::int64_t ldy) {
-    __shared__ double partial[kWaves - 1][KB][64];
+    const BlockDev* __restrict__ blocks, const DenseDev* __restrict__ denses,
+    const std::int32_t* __restrict__ widths, const double* __restrict__ lscale,
+    const double* __restrict__ rscale, const std::uint32_t* __restrict__ words,
+    const double* __restrict__ dvals, const std::int32_t* __restrict__ perm,
+    const double* __restrict__ zp, const double* __restrict__ x, std::int64_t ldx, int k,
+    double* __restrict__ y, std::int64_t ldy) {
+    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
+    __shared__ double partial[kRowWaves - 1][2][64];
+
     const TileDev tile = tiles[blockIdx.x];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p0 = blockIdx.y * KB;
     const int kk = min(KB, k - p0);
-    const bool active = lane < int(tile.rows);
+    const std::uint32_t R = tile.rows;
+    const bool active = lane < int(R);
+    std::uint32_t* stage = stage_all[wave];
 
     double acc[KB];
 #pragma unroll
@@ -202,90 +236,129 @@ __global__ __launch_bounds__(64 * kWaves) void row_tile_pass(
     const std::uint32_t e_end = tile.split[wave + 1];
     for (std::uint32_t e = tile.split[wave]; e < e_end; ++e) {
         const EntryDev ent = entries[e];
-        const std::uint32_t loc = ent.row + lane;
         if (ent.what & kDenseBit) {
+            // Diagonal block: D column-major, lanes read one column per step
+            // (coalesced); x rows staged in LDS, read as broadcasts.
             const DenseDev D = denses[ent.what & ~kDenseBit];
-            const double* col = dvals + D.off + loc;
-            for (std::uint32_t c = 0; c < D.n; ++c) {
-                const double d = active ? col[std::uint64_t(c) * D.m] : 0.0;
-                const double* xr = x + std::int64_t(perm[D.col0 + c]) * ldx + p0;
+            double* xsd = reinterpret_cast<double*>(stage);
+            constexpr std::uint32_t kCols = (kStageWords / 2) / KB < 64 ? (kStageWords / 2) / KB : 64;
+            const double* col = dvals + D.off + ent.row + lane;
+            for (std::uint32_t c0 = 0; c0 < D.n; c0 += kCols) {
+                const std::uint32_t cn = min(kCols, D.n - c0);
+                if (lane < int(cn)) {
+                    const double* xr = x + std::int64_t(perm[D.col0 + c0 + lane]) * ldx + p0;
 #pragma unroll
-                for (int p = 0; p < KB; ++p)
-                    if (p < kk) acc[p] = fma(d, xr[p], acc[p]);
+                    for (int p = 0; p < KB; ++p) xsd[lane * KB + p] = p < kk ? xr[p] : 0.0;
+                }
+                wave_lds_sync();
+                for (std::uint32_t c = 0; c < cn; ++c) {
+                    const double d = active ? col[std::uint64_t(c0 + c) * D.m] : 0.0;
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) acc[p] = fma(d, xsd[c * KB + p], acc[p]);
+                }
+                wave_lds_sync();
             }
             continue;
         }
-        const LowRankDev B = blocks[ent.what];
-        std::uint64_t woff = B.lbase;
-        float block_acc[KB];
+        const BlockDev B = blocks[ent.what];
+        float blk[KB];
 #pragma unroll
-        for (int p = 0; p < KB; ++p) block_acc[p] = 0.0f;
-        for (std::uint32_t j = 0; j < B.rank; ++j) {
-            const std::uint32_t cj = B.comp_base + j;
-            const std::uint32_t bits = widths[cj];
-            const double s = lscale[cj] * rscale[cj];
-            double z[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) z[p] = 0.0;
-            for (std::uint32_t i = B.item_begin; i < B.item_end; ++i) {
-                const double* zr = zp + (std::uint64_t(items[i].zbase) + j) * std::uint64_t(k) + p0;
-#pragma unroll
-                for (int p = 0; p < KB; ++p)
-                    if (p < kk) z[p] += zr[p];
+        for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
+        std::uint64_t first = ent.chunk;
+        for (std::uint32_t j0 = 0; j0 < B.rank;) {
+            // the components [j0, j1) whose streams fit in the stage buffer
+            std::uint32_t j1 = j0, total = 0;
+            while (j1 < B.rank) {
+                const std::uint32_t w = stream_words(R, widths[B.comp_base + j1]);
+                if (total + w > kStageWords - 4) break;
+                total += w;
+                ++j1;
             }
-            const int code = active ? decode_code(words + woff, loc, bits) : 0;
-            if (!FP32) {
-                const double cd = double(code);
+            const std::uint32_t delta = std::uint32_t(first & 3u);
+            stage_quads(stage, words + (first - delta), (total + delta + 1 + 3) >> 2, lane);
+            wave_lds_sync();
+            std::uint32_t off = delta;
+            for (std::uint32_t j = j0; j < j1; ++j) {
+                const std::uint32_t cj = B.comp_base + j;
+                const std::uint32_t bits = widths[cj];
+                const double s = lscale[cj] * rscale[cj];
+                double z[KB];
 #pragma unroll
-                for (int p = 0; p < KB; ++p) acc[p] = fma(cd, s * z[p], acc[p]);
-            } else if (bits <= 24) {
-                const float cf = float(code);
+                for (int p = 0; p < KB; ++p) z[p] = 0.0;
+                for (std::uint32_t sg = 0; sg < B.nseg; ++sg) {
+                    const double* zr =
+                        zp + (std::uint64_t(B.zbase) + sg * B.rank + j) * std::uint64_t(k) + p0;
 #pragma unroll
-                for (int p = 0; p < KB; ++p) block_acc[p] = fmaf(cf, float(s * z[p]), block_acc[p]);
-            } else {
-                const float h = float(code >> 16), l = float(code & 0xFFFF);
-#pragma unroll
-                for (int p = 0; p < KB; ++p) {
-                    const float zf = float(s * z[p]);
-                    block_acc[p] = fmaf(h, zf * 65536.0f, block_acc[p]);
-                    block_acc[p] = fmaf(l, zf, block_acc[p]);
+                    for (int p = 0; p < KB; ++p)
+                        if (p < kk) z[p] += zr[p];
                 }
+                const int code = active ? decode_code(stage + off, lane, bits) : 0;
+                if (!FP32) {
+                    const double cd = double(code);
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) acc[p] = fma(cd, s * z[p], acc[p]);
+                } else if (bits <= 24) {
+                    const float cf = float(code);
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) blk[p] = fmaf(cf, float(s * z[p]), blk[p]);
+                } else {
+                    const float h = float(code >> 16), l = float(code & 0xFFFF);
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) {
+                        const float zf = float(s * z[p]);
+                        blk[p] = fmaf(h, zf * 65536.0f, blk[p]);
+                        blk[p] = fmaf(l, zf, blk[p]);
+                    }
+                }
+                off += stream_words(R, bits);
             }
-            woff += component_words(B.m, bits);
+            first += total;
+            j0 = j1;
+            wave_lds_sync();  // the next batch restages
         }
         if (FP32) {
 #pragma unroll
-            for (int p = 0; p < KB; ++p) acc[p] += double(block_acc[p]);
+            for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
         }
     }
 
-    if (wave > 0) {
+    // Add the waves' partial rows in wave order, two right-hand sides at a time.
+    const std::int64_t yrow = active ? std::int64_t(perm[tile.row0 + lane]) * ldy + p0 : 0;
 #pragma unroll
-        for (int p = 0; p < KB; ++p) partial[wave - 1][p][lane] = acc[p];
-    }
-    __syncthreads();
-    if (wave == 0 && active) {
+    for (int pc = 0; pc < KB; pc += 2) {
+        __syncthreads();
+        if (wave > 0) {
+            partial[wave - 1][0][lane] = acc[pc];
+            partial[wave - 1][1][lane] = pc + 1 < KB ? acc[pc + 1] : 0.0;
+        }
+        __syncthreads();
+        if (wave == 0 && active) {
+            double a0 = acc[pc], a1 = pc + 1 < KB ? acc[pc + 1] : 0.0;
 #pragma unroll
-        for (int w = 0; w < kWaves - 1; ++w)
-#pragma unroll
-            for (int p = 0; p < KB; ++p) acc[p] += partial[w][p][lane];
-        double* yr = y + std::int64_t(perm[tile.row0 + lane]) * ldy + p0;
-#pragma unroll
-        for (int p = 0; p < KB; ++p)
-            if (p < kk) yr[p] = acc[p];
+            for (int w = 0; w < kRowWaves - 1; ++w) {
+                a0 += partial[w][0][lane];
+                a1 += partial[w][1][lane];
+            }
+            if (pc < kk) y[yrow + pc] = a0;
+            if (pc + 1 < kk) y[yrow + pc + 1] = a1;
+        }
     }
 }
 
-// Test hook: codes of one factor side, component-major.
-__global__ void decode_side(const std::uint32_t* __restrict__ words, std::uint64_t base,
-                            const std::int32_t* __restrict__ widths, std::uint32_t rows,
-                            std::int32_t* __restrict__ out) {
-    const std::uint32_t j = blockIdx.x;
-    std::uint64_t off = base;
-    for (std::uint32_t t = 0; t < j; ++t) off += component_words(rows, widths[t]);
-    const std::uint32_t bits = widths[j];
-    for (std::uint32_t i = threadIdx.x; i < rows; i += blockDim.x)
-        out[std::uint64_t(j) * rows + i] = decode_code(words + off, i, bits);
+// Test hook: codes of one factor side of one block, from its tile chunks,
+// written component-major in the block's own row order.
+__global__ void decode_side(const std::uint32_t* __restrict__ words,
+                            const ChunkRef* __restrict__ chunks,
+                            const std::int32_t* __restrict__ widths, std::uint32_t rank,
+                            std::uint32_t rows, std::int32_t* __restrict__ out) {
+    const ChunkRef ch = chunks[blockIdx.x];
+    std::uint64_t off = ch.chunk;
+    for (std::uint32_t j = 0; j < rank; ++j) {
+        const std::uint32_t bits = widths[j];
+        for (std::uint32_t i = threadIdx.x; i < ch.count; i += blockDim.x)
+            out[std::uint64_t(j) * rows + ch.first + i] = decode_code(words + off, i, bits);
+        off += stream_words(ch.count, bits);
+    }
 }
 
 // Event slots of the next profiled apply, or null when not profiling.
@@ -301,15 +374,15 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     const unsigned passes = unsigned((k + KB - 1) / KB);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
-    if (s.n_items) {
-        hipLaunchKernelGGL((right_factor_pass<KB, FP32>), dim3(s.n_items, passes), dim3(64), 0,
-                           stream, s.items, s.lowrank, s.widths, s.words, s.perm, x, ldx, k, zp);
+    if (s.n_units) {
+        hipLaunchKernelGGL((right_factor_pass<KB, FP32>), dim3(s.n_units, passes), dim3(64), 0,
+                           stream, s.units, s.blocks, s.widths, s.words, s.perm, x, ldx, k, zp);
         hip_check(hipGetLastError(), "right-factor pass launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
-    hipLaunchKernelGGL((row_tile_pass<KB, FP32>), dim3(s.n_tiles, passes), dim3(64 * kWaves), 0,
-                       stream, s.tiles, s.entries, s.lowrank, s.denses, s.items, s.widths,
-                       s.lscale, s.rscale, s.words, s.dense, s.perm, zp, x, ldx, k, y, ldy);
+    hipLaunchKernelGGL((row_tile_pass<KB, FP32>), dim3(s.n_tiles, passes), dim3(64 * kRowWaves),
+                       0, stream, s.tiles, s.entries, s.blocks, s.denses, s.widths, s.lscale,
+                       s.rscale, s.words, s.dense, s.perm, zp, x, ldx, k, y, ldy);
     hip_check(hipGetLastError(), "row-tile pass launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
 }
@@ -334,11 +407,45 @@ template <class T> T* upload(const std::vector<T>& host, std::uint64_t& bytes) {
     return p;
 }
 
-void require(bool ok, const char* message) {
-    if (!ok) throw std::invalid_argument(message);
+// Tiles of at most kTile indices between consecutive cuts.
+std::vector<std::pair<std::uint64_t, std::uint64_t>> make_tiles(std::vector<std::uint64_t> cuts) {
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    std::vector<std::pair<std::uint64_t, std::uint64_t>> tiles;
+    for (std::size_t i = 0; i + 1 < cuts.size(); ++i)
+        for (std::uint64_t r = cuts[i]; r < cuts[i + 1]; r += kTile)
+            tiles.emplace_back(r, std::min<std::uint64_t>(r + kTile, cuts[i + 1]));
+    return tiles;
+}
+
+// Index of the first tile starting at or after `at`.
+std::size_t first_tile(const std::vector<std::pair<std::uint64_t, std::uint64_t>>& tiles,
+                       std::uint64_t at) {
+    return std::lower_bound(tiles.begin(), tiles.end(), at,
+                            [](const auto& t, std::uint64_t r) { return t.first < r; }) -
+           tiles.begin();
 }
 
 }  // namespace
+
+void append_codes(const std::uint32_t* stream, std::uint64_t stream_words_count,
+                  std::uint64_t first, std::uint64_t count, int bits,
+                  std::vector<std::uint32_t>& out) {
+    const std::uint64_t bit0 = first * std::uint64_t(bits);
+    const std::uint64_t nbits = count * std::uint64_t(bits);
+    const std::uint64_t nw = (nbits + 31) / 32;
+    for (std::uint64_t o = 0; o < nw; ++o) {
+        const std::uint64_t src = bit0 + 32 * o;
+        const std::uint64_t w = src >> 5;
+        const unsigned sh = unsigned(src & 31);
+        std::uint64_t pair = stream[w];
+        if (sh && w + 1 < stream_words_count) pair |= std::uint64_t(stream[w + 1]) << 32;
+        std::uint32_t v = std::uint32_t(pair >> sh);
+        const std::uint64_t left = nbits - 32 * o;
+        if (left < 32) v &= (std::uint32_t{1} << left) - 1u;
+        out.push_back(v);
+    }
+}
 
 int device_count() {
     int n = 0;
@@ -346,14 +453,11 @@ int device_count() {
     return n;
 }
 
-DeviceStore* create_device_store(const qh_store_desc& d, int device) {
+Plan build_plan(const qh_store_desc& d) {
     require(d.order > 0 && d.perm && d.nblocks > 0 && d.rect && d.is_dense && d.rank,
             "incomplete store description");
     require(d.order < (std::uint64_t{1} << 26),
             "matrix order exceeds the device store limit (2^26)");
-    const int count = device_count();
-    if (count == 0) throw std::runtime_error("no CUDA device is available (no HIP device found)");
-    require(device >= 0 && device < count, "invalid device ordinal");
     const std::uint64_t n = d.order;
 
     std::vector<std::int32_t> perm(n);
@@ -365,47 +469,55 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
             perm[i] = static_cast<std::int32_t>(d.perm[i]);
         }
     }
+    std::vector<std::uint64_t> row_cuts{0, n}, col_cuts{0, n};
+    for (std::uint64_t b = 0; b < d.nblocks; ++b) {
+        const std::uint64_t* r = d.rect + 4 * b;
+        require(r[0] < r[1] && r[1] <= n && r[2] < r[3] && r[3] <= n, "invalid block rectangle");
+        row_cuts.push_back(r[0]);
+        row_cuts.push_back(r[1]);
+        col_cuts.push_back(r[2]);
+        col_cuts.push_back(r[3]);
+    }
+    const auto row_tiles = make_tiles(row_cuts);
+    const auto col_tiles = make_tiles(col_cuts);
 
-    auto* s = new DeviceStore;
-    s->device = device;
-    s->order = n;
+    Plan plan;
+    Plan* s = &plan;
+    plan.order = n;
     std::vector<std::uint32_t> words;
     std::vector<std::int32_t> widths;
     std::vector<double> lsc, rsc, dvals;
-    std::vector<LowRankDev> lowrank;
+    std::vector<BlockDev> blocks;
     std::vector<DenseDev> denses;
-    std::vector<ItemDev> items;
+    std::vector<UnitDev> units;
+    std::vector<std::vector<EntryDev>> per_tile(row_tiles.size());
+    std::vector<std::vector<std::uint64_t>> per_cost(row_tiles.size());
     s->block_to_lowrank.assign(d.nblocks, -1);
-    s->block_rows.resize(d.nblocks);
-    s->block_cols.resize(d.nblocks);
     std::uint64_t codes = 0, lr_flops = 0, dn_flops = 0, z_rows = 0;
     std::uint64_t n_lowrank_all = 0, left_words = 0, right_words = 0;
-    // Entry candidates in insertion order: (row0, row1, what).
-    struct Cover {
-        std::uint64_t row0, row1;
-        std::uint32_t what;
-        std::uint64_t cost;
+
+    auto pad4 = [&words] {
+        while (words.size() & 3u) words.push_back(0u);
     };
-    std::vector<Cover> covers;
-    covers.reserve(d.nblocks);
     for (std::uint64_t b = 0; b < d.nblocks; ++b) {
         const std::uint64_t r0 = d.rect[4 * b], r1 = d.rect[4 * b + 1];
         const std::uint64_t c0 = d.rect[4 * b + 2], c1 = d.rect[4 * b + 3];
-        require(r0 < r1 && r1 <= n && c0 < c1 && c1 <= n, "invalid block rectangle");
         const std::uint64_t m = r1 - r0, nc = c1 - c0;
-        s->block_rows[b] = m;
-        s->block_cols[b] = nc;
+        const std::size_t t0 = first_tile(row_tiles, r0);
         if (d.is_dense[b]) {
             require(d.dense && d.dense[b], "dense block without values");
-            DenseDev D{std::uint32_t(r0), std::uint32_t(c0), std::uint32_t(m), std::uint32_t(nc),
-                       dvals.size()};
+            const std::uint32_t di = std::uint32_t(denses.size());
+            denses.push_back({std::uint32_t(r0), std::uint32_t(c0), std::uint32_t(m),
+                              std::uint32_t(nc), dvals.size()});
             const double* v = d.dense[b];
             const std::size_t at = dvals.size();
             dvals.resize(at + m * nc);
             for (std::uint64_t i = 0; i < m; ++i)  // row-major in, column-major out
                 for (std::uint64_t c = 0; c < nc; ++c) dvals[at + c * m + i] = v[i * nc + c];
-            covers.push_back({r0, r1, kDenseBit | std::uint32_t(denses.size()), nc});
-            denses.push_back(D);
+            for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t) {
+                per_tile[t].push_back({kDenseBit | di, std::uint32_t(row_tiles[t].first - r0), 0});
+                per_cost[t].push_back(3 * nc);
+            }
             dn_flops += 2 * m * nc;
             continue;
         }
@@ -416,13 +528,9 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
         require(d.widths && d.widths[b] && d.lscale && d.lscale[b] && d.rscale && d.rscale[b] &&
                     d.lwords && d.lwords[b] && d.rwords && d.rwords[b],
                 "low-rank block without factors");
-        LowRankDev L{};
-        L.row0 = std::uint32_t(r0);
-        L.col0 = std::uint32_t(c0);
-        L.m = std::uint32_t(m);
-        L.n = std::uint32_t(nc);
-        L.rank = std::uint32_t(r);
-        L.comp_base = std::uint32_t(widths.size());
+        const std::uint32_t li = std::uint32_t(blocks.size());
+        BlockDev B{std::uint32_t(r), std::uint32_t(widths.size()), std::uint32_t(z_rows), 0};
+        std::vector<std::uint64_t> loff(r), roff(r), lcnt(r), rcnt(r);
         std::uint64_t lw = 0, rw = 0;
         for (std::uint64_t j = 0; j < r; ++j) {
             const int bits = d.widths[b][j];
@@ -430,127 +538,241 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
             widths.push_back(bits);
             lsc.push_back(d.lscale[b][j]);
             rsc.push_back(d.rscale[b][j]);
-            lw += (m * bits + 31) / 32;
-            rw += (nc * bits + 31) / 32;
+            loff[j] = lw;
+            roff[j] = rw;
+            lcnt[j] = (m * bits + 31) / 32;
+            rcnt[j] = (nc * bits + 31) / 32;
+            lw += lcnt[j];
+            rw += rcnt[j];
         }
         left_words += lw;
         right_words += rw;
-        L.lbase = words.size();
-        words.insert(words.end(), d.lwords[b], d.lwords[b] + lw);
-        L.rbase = words.size();
-        words.insert(words.end(), d.rwords[b], d.rwords[b] + rw);
-        L.item_begin = std::uint32_t(items.size());
-        for (std::uint64_t c = 0; c < nc; c += kItemCols) {
-            items.push_back({std::uint32_t(lowrank.size()), std::uint32_t(c),
-                             std::uint32_t(std::min<std::uint64_t>(nc, c + kItemCols)),
-                             std::uint32_t(z_rows)});
-            z_rows += r;
+        // left factor: one chunk per row tile
+        s->left_chunks.emplace_back();
+        for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t) {
+            const std::uint64_t a = row_tiles[t].first - r0;
+            const std::uint64_t rows = row_tiles[t].second - row_tiles[t].first;
+            const std::uint64_t at = words.size();
+            for (std::uint64_t j = 0; j < r; ++j)
+                append_codes(d.lwords[b] + loff[j], lcnt[j], a, rows, d.widths[b][j], words);
+            const std::uint64_t chunk_words = words.size() - at;
+            pad4();
+            s->left_chunks.back().push_back({std::uint32_t(a), std::uint32_t(rows), at});
+            per_tile[t].push_back({li, std::uint32_t(a), at});
+            per_cost[t].push_back(8 * r + chunk_words / 16);
         }
-        L.item_end = std::uint32_t(items.size());
-        s->block_to_lowrank[b] = std::int64_t(lowrank.size());
-        covers.push_back({r0, r1, std::uint32_t(lowrank.size()), 4 + 3 * r});
-        lowrank.push_back(L);
+        // right factor: one chunk (and one work unit) per column tile
+        s->right_chunks.emplace_back();
+        for (std::size_t t = first_tile(col_tiles, c0);
+             t < col_tiles.size() && col_tiles[t].first < c1; ++t) {
+            const std::uint64_t a = col_tiles[t].first - c0;
+            const std::uint64_t cols = col_tiles[t].second - col_tiles[t].first;
+            const std::uint64_t at = words.size();
+            for (std::uint64_t j = 0; j < r; ++j)
+                append_codes(d.rwords[b] + roff[j], rcnt[j], a, cols, d.widths[b][j], words);
+            pad4();
+            s->right_chunks.back().push_back({std::uint32_t(a), std::uint32_t(cols), at});
+            units.push_back({li, std::uint32_t(col_tiles[t].first), std::uint32_t(cols),
+                             std::uint32_t(z_rows + B.nseg * r), at});
+            ++B.nseg;
+        }
+        z_rows += std::uint64_t(B.nseg) * r;
+        s->block_to_lowrank[b] = li;
+        blocks.push_back(B);
         codes += (m + nc) * r;
         lr_flops += 2 * r * (m + nc);
     }
     require(z_rows < (std::uint64_t{1} << 31), "store too large for one device plan");
-    words.resize(words.size() + 4, 0u);  // pair reads past the last code
+    const std::uint64_t stored_words = words.size();
+    words.resize(words.size() + 8, 0u);  // reads one word past the last chunk
 
-    // Row tiles: slabs between distinct row boundaries, cut into <= 64 rows.
-    std::vector<std::uint64_t> cuts{0, n};
-    for (std::uint64_t b = 0; b < d.nblocks; ++b) {
-        cuts.push_back(d.rect[4 * b]);
-        cuts.push_back(d.rect[4 * b + 1]);
-    }
-    std::sort(cuts.begin(), cuts.end());
-    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
-    std::vector<TileDev> tiles;
-    for (std::size_t sl = 0; sl + 1 < cuts.size(); ++sl)
-        for (std::uint64_t r = cuts[sl]; r < cuts[sl + 1]; r += kTileRows) {
-            TileDev t{};
-            t.row0 = std::uint32_t(r);
-            t.rows = std::uint32_t(std::min<std::uint64_t>(kTileRows, cuts[sl + 1] - r));
-            tiles.push_back(t);
-        }
-    std::vector<std::vector<EntryDev>> per_tile(tiles.size());
-    std::vector<std::vector<std::uint64_t>> per_cost(tiles.size());
-    for (const Cover& c : covers) {
-        auto first = std::lower_bound(tiles.begin(), tiles.end(), c.row0,
-                                      [](const TileDev& t, std::uint64_t r) { return t.row0 < r; });
-        for (auto t = first; t != tiles.end() && t->row0 < c.row1; ++t) {
-            const std::size_t ti = std::size_t(t - tiles.begin());
-            per_tile[ti].push_back({c.what, std::uint32_t(t->row0 - c.row0)});
-            per_cost[ti].push_back(c.cost);
-        }
-    }
+    std::vector<TileDev> tiles(row_tiles.size());
     std::vector<EntryDev> entries;
-    for (std::size_t ti = 0; ti < tiles.size(); ++ti) {
-        const auto& list = per_tile[ti];
-        const auto& cost = per_cost[ti];
+    for (std::size_t t = 0; t < row_tiles.size(); ++t) {
+        TileDev& T = tiles[t];
+        T.row0 = std::uint32_t(row_tiles[t].first);
+        T.rows = std::uint32_t(row_tiles[t].second - row_tiles[t].first);
+        const auto& list = per_tile[t];
+        const auto& cost = per_cost[t];
         const std::uint32_t base = std::uint32_t(entries.size());
         entries.insert(entries.end(), list.begin(), list.end());
-        // Contiguous split of the entries into kWaves runs of similar cost.
+        // Contiguous split of the entries into kRowWaves runs of similar cost.
         std::uint64_t total = 0;
         for (auto c : cost) total += c;
         std::uint32_t e = 0;
         std::uint64_t run = 0;
-        tiles[ti].split[0] = base;
-        for (int w = 1; w < kWaves; ++w) {
-            const std::uint64_t goal = total * std::uint64_t(w) / kWaves;
+        T.split[0] = base;
+        for (int w = 1; w < kRowWaves; ++w) {
+            const std::uint64_t goal = total * std::uint64_t(w) / kRowWaves;
             while (e < list.size() && run + cost[e] / 2 < goal) run += cost[e++];
-            tiles[ti].split[w] = base + e;
+            T.split[w] = base + e;
         }
-        tiles[ti].split[kWaves] = base + std::uint32_t(list.size());
+        T.split[kRowWaves] = base + std::uint32_t(list.size());
     }
 
-    try {
-        DeviceScope scope(device);
-        std::uint64_t bytes = 0;
-        s->perm = upload(perm, bytes);
-        s->words = upload(words, bytes);
-        s->widths = upload(widths, bytes);
-        s->lscale = upload(lsc, bytes);
-        s->rscale = upload(rsc, bytes);
-        s->dense = upload(dvals, bytes);
-        s->lowrank = upload(lowrank, bytes);
-        s->denses = upload(denses, bytes);
-        s->items = upload(items, bytes);
-        s->tiles = upload(tiles, bytes);
-        s->entries = upload(entries, bytes);
-        s->device_bytes = bytes;
-    } catch (...) {
-        destroy_device_store(s);
-        throw;
-    }
-    s->n_words = words.size() - 4;
-    s->n_comp = widths.size();
-    s->n_dense_values = dvals.size();
-    s->n_lowrank = std::uint32_t(lowrank.size());
-    s->n_dense = std::uint32_t(denses.size());
-    s->n_items = std::uint32_t(items.size());
-    s->n_tiles = std::uint32_t(tiles.size());
-    s->n_entries = std::uint32_t(entries.size());
-    s->z_rows = z_rows;
-
-    qh_store_stats& st = s->stats;
+    qh_store_stats& st = plan.stats;
     st.order = n;
     st.nblocks = d.nblocks;
     st.ndense = denses.size();
     st.nlowrank = n_lowrank_all;
     st.ncomponents = widths.size();
-    st.code_words = s->n_words;
+    st.code_words = left_words + right_words;
     st.left_code_words = left_words;
     st.right_code_words = right_words;
     st.codes = codes;
     st.dense_values = dvals.size();
-    st.device_bytes = s->device_bytes;
     st.lowrank_flops_per_rhs = lr_flops;
     st.dense_flops_per_rhs = dn_flops;
     st.tiles = tiles.size();
-    st.items = items.size();
+    st.items = units.size();
     st.entries = entries.size();
-    st.device = device;
+
+    plan.perm = std::move(perm);
+    plan.words = std::move(words);
+    plan.widths = std::move(widths);
+    plan.lscale = std::move(lsc);
+    plan.rscale = std::move(rsc);
+    plan.dense = std::move(dvals);
+    plan.blocks = std::move(blocks);
+    plan.denses = std::move(denses);
+    plan.units = std::move(units);
+    plan.tiles = std::move(tiles);
+    plan.entries = std::move(entries);
+    plan.z_rows = z_rows;
+    plan.stored_words = stored_words;
+    return plan;
+}
+
+DeviceStore* create_device_store(const qh_store_desc& d, int device) {
+    const int count = device_count();
+    if (count == 0) throw std::runtime_error("no CUDA device is available (no HIP device found)");
+    require(device >= 0 && device < count, "invalid device ordinal");
+    Plan plan = build_plan(d);
+    auto* s = new DeviceStore;
+    s->device = device;
+    s->order = plan.order;
+    try {
+        DeviceScope scope(device);
+        std::uint64_t bytes = 0;
+        s->perm = upload(plan.perm, bytes);
+        s->words = upload(plan.words, bytes);
+        s->widths = upload(plan.widths, bytes);
+        s->lscale = upload(plan.lscale, bytes);
+        s->rscale = upload(plan.rscale, bytes);
+        s->dense = upload(plan.dense, bytes);
+        s->blocks = upload(plan.blocks, bytes);
+        s->denses = upload(plan.denses, bytes);
+        s->units = upload(plan.units, bytes);
+        s->tiles = upload(plan.tiles, bytes);
+        s->entries = upload(plan.entries, bytes);
+        s->device_bytes = bytes;
+    } catch (...) {
+        destroy_device_store(s);
+        throw;
+    }
+    s->n_words = plan.stored_words;
+    s->n_comp = plan.widths.size();
+    s->n_dense_values = plan.dense.size();
+    s->n_lowrank = std::uint32_t(plan.blocks.size());
+    s->n_dense = std::uint32_t(plan.denses.size());
+    s->n_units = std::uint32_t(plan.units.size());
+    s->n_tiles = std::uint32_t(plan.tiles.size());
+    s->n_entries = std::uint32_t(plan.entries.size());
+    s->z_rows = plan.z_rows;
+    s->block_to_lowrank = std::move(plan.block_to_lowrank);
+    s->left_chunks = std::move(plan.left_chunks);
+    s->right_chunks = std::move(plan.right_chunks);
+    for (const auto& B : plan.blocks) {
+        s->comp_base.push_back(B.comp_base);
+        s->ranks.push_back(B.rank);
+    }
+    s->stats = plan.stats;
+    s->stats.device_bytes = s->device_bytes;
+    s->stats.device = device;
     return s;
+}
+
+// Bounds-checked element access for the host emulation.
+template <class V> const typename V::value_type& at(const V& v, std::uint64_t i) {
+    if (i >= v.size()) throw std::out_of_range("device plan index out of range");
+    return v[i];
+}
+
+void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
+    const std::uint64_t n = pl.order;
+    std::vector<double> zp(pl.z_rows * std::uint64_t(k), 0.0);
+    auto code_at = [&](std::uint64_t seg_word, std::uint32_t idx, std::uint32_t bits) {
+        const std::uint32_t pos = idx * bits;
+        const std::uint64_t w = seg_word + (pos >> 5);
+        const std::uint64_t pair = std::uint64_t(at(pl.words, w)) |
+                                   (std::uint64_t(at(pl.words, w + 1)) << 32);
+        const std::uint64_t raw = (pair >> (pos & 31u)) & ((std::uint64_t{1} << bits) - 1);
+        const std::uint64_t sign = std::uint64_t{1} << (bits - 1);
+        return double(std::int64_t(raw ^ sign) - std::int64_t(sign));
+    };
+    // Pass 1: every unit, the components in passes of kPassComponents.
+    for (const UnitDev& u : pl.units) {
+        const BlockDev& B = at(pl.blocks, u.block);
+        if (u.cols == 0 || u.cols > std::uint32_t(kTile)) throw std::out_of_range("unit width");
+        std::uint64_t done = 0;
+        for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
+            const std::uint32_t jn = std::min<std::uint32_t>(kPassComponents, B.rank - j0);
+            std::uint64_t total = 0;
+            for (std::uint32_t g = 0; g < jn; ++g)
+                total += stream_words(u.cols, at(pl.widths, B.comp_base + j0 + g));
+            const std::uint64_t first = u.chunk + done;
+            const std::uint64_t delta = first & 3u;
+            const std::uint64_t quads = (total + delta + 1 + 3) >> 2;
+            if (4 * quads > std::uint64_t(kStageWords + 8)) throw std::out_of_range("stage size");
+            at(pl.words, first - delta + 4 * quads - 1);  // staged range is readable
+            std::uint64_t seg = 0;
+            for (std::uint32_t g = 0; g < jn; ++g) {
+                const std::uint32_t j = j0 + g;
+                const std::uint32_t bits = at(pl.widths, B.comp_base + j);
+                for (std::int64_t p = 0; p < k; ++p) {
+                    double sum = 0.0;
+                    for (std::uint32_t c = 0; c < u.cols; ++c)
+                        sum += code_at(first + seg, c, bits) *
+                               x[std::int64_t(at(pl.perm, u.col0 + c)) * k + p];
+                    zp.at((std::uint64_t(u.zrow) + j) * std::uint64_t(k) + p) = sum;
+                }
+                seg += stream_words(u.cols, bits);
+            }
+            done += total;
+        }
+    }
+    // Pass 2: every tile, every wave's entries.
+    for (const TileDev& T : pl.tiles) {
+        if (T.rows == 0 || T.rows > std::uint32_t(kTile)) throw std::out_of_range("tile rows");
+        for (std::int64_t p = 0; p < k; ++p)
+            for (std::uint32_t i = 0; i < T.rows; ++i) {
+                double acc = 0.0;
+                for (int w = 0; w < kRowWaves; ++w)
+                    for (std::uint32_t e = T.split[w]; e < T.split[w + 1]; ++e) {
+                        const EntryDev& ent = at(pl.entries, e);
+                        if (ent.what & kDenseBit) {
+                            const DenseDev& D = at(pl.denses, ent.what & ~kDenseBit);
+                            for (std::uint32_t c = 0; c < D.n; ++c)
+                                acc += at(pl.dense, D.off + std::uint64_t(c) * D.m + ent.row + i) *
+                                       x[std::int64_t(at(pl.perm, D.col0 + c)) * k + p];
+                            continue;
+                        }
+                        const BlockDev& B = at(pl.blocks, ent.what);
+                        std::uint64_t off = ent.chunk;
+                        for (std::uint32_t j = 0; j < B.rank; ++j) {
+                            const std::uint32_t cj = B.comp_base + j;
+                            const std::uint32_t bits = at(pl.widths, cj);
+                            double z = 0.0;
+                            for (std::uint32_t sg = 0; sg < B.nseg; ++sg)
+                                z += zp.at((std::uint64_t(B.zbase) + sg * B.rank + j) * k + p);
+                            acc += code_at(off, i, bits) * (at(pl.lscale, cj) * at(pl.rscale, cj) * z);
+                            off += stream_words(T.rows, bits);
+                        }
+                    }
+                y[std::int64_t(at(pl.perm, T.row0 + i)) * k + p] = acc;
+            }
+    }
+    (void)n;
 }
 
 void destroy_device_store(DeviceStore* s) {
@@ -562,8 +784,8 @@ void destroy_device_store(DeviceStore* s) {
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
                     static_cast<void*>(s->widths), static_cast<void*>(s->lscale),
                     static_cast<void*>(s->rscale), static_cast<void*>(s->dense),
-                    static_cast<void*>(s->lowrank), static_cast<void*>(s->denses),
-                    static_cast<void*>(s->items), static_cast<void*>(s->tiles),
+                    static_cast<void*>(s->blocks), static_cast<void*>(s->denses),
+                    static_cast<void*>(s->units), static_cast<void*>(s->tiles),
                     static_cast<void*>(s->entries)})
         if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
@@ -644,14 +866,24 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
     require(out != nullptr, "null output pointer");
     const std::int64_t li = s.block_to_lowrank[block];
     if (li < 0) return;  // dense or rank 0: nothing to decode
-    LowRankDev L;
+    const auto& chunks = side == 0 ? s.left_chunks[li] : s.right_chunks[li];
+    std::uint32_t rows = 0;
+    for (const auto& c : chunks) rows = std::max(rows, c.first + c.count);
     DeviceScope scope(s.device);
-    hip_check(hipMemcpy(&L, s.lowrank + li, sizeof(L), hipMemcpyDeviceToHost), "hipMemcpy");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(decode_side, dim3(L.rank), dim3(256), 0, st, s.words,
-                       side == 0 ? L.lbase : L.rbase, s.widths + L.comp_base,
-                       side == 0 ? L.m : L.n, out);
-    hip_check(hipGetLastError(), "decode launch");
+    ChunkRef* dev = nullptr;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&dev), chunks.size() * sizeof(ChunkRef)),
+              "hipMalloc");
+    hipError_t e = hipMemcpy(dev, chunks.data(), chunks.size() * sizeof(ChunkRef),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(decode_side, dim3(unsigned(chunks.size())), dim3(64), 0, st, s.words,
+                           dev, s.widths + s.comp_base[li], s.ranks[li], rows, out);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(dev);
+    hip_check(e, "decode launch");
 }
 
 }  // namespace qhamd

@@ -1,4 +1,15 @@
 // Device-resident packed store and the apply plan (internal to the library).
+//
+// Layout in HBM ("tile-major"): the matrix rows are cut into row tiles of at
+// most kTile rows that never cross a block boundary, and the columns into
+// column tiles the same way.  Each low-rank block's left factor is stored as
+// one chunk per row tile it covers and its right factor as one chunk per
+// column tile: a chunk holds, for every component j, the codes of that
+// tile's rows (or columns) as a word-aligned LSB-first stream of
+// ceil(R*b_j/32) words -- the reference's packed format (core.cpp:235-267)
+// cut at tile boundaries -- and is padded to a multiple of four words so
+// that a wave can stream it with 16-byte loads.  When a tile starts on a
+// multiple of 32 codes the chunk's words are the reference's words verbatim.
 #pragma once
 
 #include <cstdint>
@@ -8,50 +19,51 @@
 
 namespace qhamd {
 
-// Row tiles are owned by one workgroup of kWaves waves; a tile has at most
-// kTileRows rows (one per lane) and never crosses a row-slab boundary, so
-// every block covering one row of a tile covers all of them.
-constexpr int kTileRows = 64;
-constexpr int kWaves = 4;
-// Columns of a low-rank block reduced by one wave in the right-factor pass.
-constexpr int kItemCols = 256;
+constexpr int kTile = 64;          // rows per row tile, columns per column tile
+constexpr int kRowWaves = 8;       // waves per row-tile workgroup
+constexpr int kStageWords = 1024;  // LDS words a wave stages at a time
+constexpr int kPassComponents = 16;  // components per right-factor pass (4 lanes each)
 
-// One off-diagonal block with rank > 0.  lbase/rbase: first word of its left
-// and right factor in the word array; components follow the packed format
-// (component j at the previous one's end, ceil(rows*b/32) words each).
-struct LowRankDev {
-    std::uint32_t row0, col0, m, n;
+// Low-rank block with rank > 0.  Right-factor partial sums of column tile s,
+// component j live in workspace row zbase + s * rank + j.
+struct BlockDev {
     std::uint32_t rank, comp_base;
-    std::uint32_t item_begin, item_end;
-    std::uint64_t lbase, rbase;
+    std::uint32_t zbase, nseg;
 };
 
-// A diagonal block: m x n FP64 stored column-major at values[off].
+// Right-factor unit: block `block` restricted to columns [col0, col0+cols).
+struct UnitDev {
+    std::uint32_t block, col0, cols, zrow;
+    std::uint64_t chunk;  // first word, multiple of 4
+};
+
+// Row-tile entry: kDenseBit | dense index, or low-rank block index.
+struct EntryDev {
+    std::uint32_t what;
+    std::uint32_t row;    // dense: first tile row relative to the block
+    std::uint64_t chunk;  // low-rank: first word of the block's chunk for this tile
+};
+constexpr std::uint32_t kDenseBit = 0x80000000u;
+
+// Row tile: rows [row0, row0+rows); entries [split[w], split[w+1]) are summed
+// by wave w and the waves are combined in order 0..kRowWaves-1.
+struct TileDev {
+    std::uint32_t row0, rows;
+    std::uint32_t split[kRowWaves + 1];
+    std::uint32_t pad;
+};
+
+// Diagonal block: m x n FP64 stored column-major at values[off].
 struct DenseDev {
     std::uint32_t row0, col0, m, n;
     std::uint64_t off;
 };
 
-// Right-factor work item: columns [c0, c1) of block `block`; its partial
-// dot products go to zp[(zbase + j) * k + p].
-struct ItemDev {
-    std::uint32_t block, c0, c1, zbase;
+// Host record of one chunk (test hook and statistics).
+struct ChunkRef {
+    std::uint32_t first, count;  // rows/columns of the block it holds
+    std::uint64_t chunk;
 };
-
-// Row tile: rows [row0, row0+rows); entries [split[w], split[w+1]) are summed
-// by wave w, waves are combined in order 0..kWaves-1.
-struct TileDev {
-    std::uint32_t row0, rows;
-    std::uint32_t split[kWaves + 1];
-    std::uint32_t pad;
-};
-
-// Entry of a tile: bit 31 set = dense block, low 31 bits = index into the
-// dense or low-rank array; `row` = first tile row relative to the block.
-struct EntryDev {
-    std::uint32_t what, row;
-};
-constexpr std::uint32_t kDenseBit = 0x80000000u;
 
 struct DeviceStore {
     int device = 0;
@@ -63,18 +75,20 @@ struct DeviceStore {
     double* lscale = nullptr;
     double* rscale = nullptr;
     double* dense = nullptr;
-    LowRankDev* lowrank = nullptr;
+    BlockDev* blocks = nullptr;
     DenseDev* denses = nullptr;
-    ItemDev* items = nullptr;
+    UnitDev* units = nullptr;
     TileDev* tiles = nullptr;
     EntryDev* entries = nullptr;
     // sizes
     std::uint64_t n_words = 0, n_comp = 0, n_dense_values = 0;
-    std::uint32_t n_lowrank = 0, n_dense = 0, n_items = 0, n_tiles = 0, n_entries = 0;
-    std::uint64_t z_rows = 0;  // sum of rank over items: workspace = z_rows * k doubles
+    std::uint32_t n_lowrank = 0, n_dense = 0, n_units = 0, n_tiles = 0, n_entries = 0;
+    std::uint64_t z_rows = 0;  // workspace = z_rows * k doubles
     // host mirrors needed by entry points
-    std::vector<std::int64_t> block_to_lowrank;  // insertion index -> lowrank index or -1
-    std::vector<std::uint64_t> block_rows, block_cols;
+    std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
+    std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;  // per low-rank block
+    std::vector<std::uint32_t> comp_base;                          // per low-rank block
+    std::vector<std::uint32_t> ranks;                              // per low-rank block
     qh_store_stats stats{};
     std::uint64_t device_bytes = 0;
     // Optional per-pass event timing (qh_profile_begin / qh_profile_end).
@@ -86,10 +100,30 @@ struct DeviceStore {
     mutable Profile profile;
 };
 
-void profile_begin(DeviceStore& s, std::uint64_t max_applies);
-qh_profile profile_end(DeviceStore& s);
+// Everything the device store holds, built on the host before the upload.
+struct Plan {
+    std::uint64_t order = 0;
+    std::vector<std::int32_t> perm;
+    std::vector<std::uint32_t> words;  // chunks + 8 words of padding
+    std::vector<std::int32_t> widths;
+    std::vector<double> lscale, rscale, dense;
+    std::vector<BlockDev> blocks;
+    std::vector<DenseDev> denses;
+    std::vector<UnitDev> units;
+    std::vector<TileDev> tiles;
+    std::vector<EntryDev> entries;
+    std::vector<std::int64_t> block_to_lowrank;
+    std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;
+    std::uint64_t z_rows = 0, stored_words = 0;
+    qh_store_stats stats{};
+};
 
 // Throw std::invalid_argument / std::runtime_error like the host store.
+Plan build_plan(const qh_store_desc& desc);
+// Runs the two passes' index arithmetic on the host against the plan's
+// arrays, with every array access bounds-checked (std::out_of_range), and
+// returns y = H x in FP64.  A check of the layout without a device.
+void emulate_plan(const Plan& plan, const double* x, std::int64_t k, double* y);
 DeviceStore* create_device_store(const qh_store_desc& desc, int device);
 void destroy_device_store(DeviceStore* s);
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k);
@@ -99,5 +133,12 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
 void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int32_t* out,
                   void* stream);
 int device_count();
+void profile_begin(DeviceStore& s, std::uint64_t max_applies);
+qh_profile profile_end(DeviceStore& s);
+
+// Appends codes [first, first+count) of one component stream of width `bits`
+// to `out` as a fresh word-aligned stream (host side of the upload).
+void append_codes(const std::uint32_t* stream, std::uint64_t stream_words, std::uint64_t first,
+                  std::uint64_t count, int bits, std::vector<std::uint32_t>& out);
 
 }  // namespace qhamd
