@@ -124,6 +124,32 @@ qh_status qh_apply(const qh_store *store, const double *x, int64_t ldx, int64_t 
                    int64_t ldy, int precision, void *stream, void *workspace,
                    uint64_t workspace_bytes);
 
+/* ---- batches of resident stores (no counterpart in the reference) -------
+ * Many stores of uneven order, rank and right-hand-side count applied
+ * together: one launch pair per right-hand-side class (k rounded up to
+ * 1, 2, 4, 8 or 16) covers every member of the class.  Member i reads its
+ * order_i x k_i row-major block at x + offsets[i] and writes y at the same
+ * offset; the stores must live on one device.  A member's result is bitwise
+ * the one qh_apply gives for it alone. */
+typedef struct qh_batch qh_batch;
+qh_status qh_batch_create(const qh_store *const *stores, const int64_t *k, uint64_t count,
+                          qh_batch **out);
+qh_status qh_batch_destroy(qh_batch *batch);
+/* offsets: count entries (doubles); total: doubles in x (and y). */
+qh_status qh_batch_layout(const qh_batch *batch, uint64_t *offsets, uint64_t *total,
+                          uint64_t *workspace_bytes);
+qh_status qh_batch_apply(const qh_batch *batch, const double *x, double *y, int precision,
+                         void *stream, void *workspace, uint64_t workspace_bytes);
+
+/* The batch apply captured once into a HIP graph (fixed x, y, workspace and
+ * precision) and replayed with one qh_graph_launch. */
+typedef struct qh_graph qh_graph;
+qh_status qh_batch_graph_create(const qh_batch *batch, const double *x, double *y,
+                                int precision, void *workspace, uint64_t workspace_bytes,
+                                qh_graph **out);
+qh_status qh_graph_launch(const qh_graph *graph, void *stream);
+qh_status qh_graph_destroy(qh_graph *graph);
+
 /* Per-pass timing of qh_apply on the stream the passes are launched on.
  * qh_profile_begin arms hipEvents around each of the two passes of the next
  * `max_applies` calls on this store (the events are created here, not in the

@@ -378,18 +378,18 @@ class NativeHMatrix:
     def finalized(self) -> bool:
         return bool(self._info().finalized)
 
+    def block(self, index: int) -> BlockInfo:
+        """Metadata of one block (insertion order)."""
+        info = _BlockInfo()
+        check(_lib.qh_host_block_info(self._h, int(index), C.byref(info)))
+        return BlockInfo(
+            Rectangle(info.row0, info.row1, info.col0, info.col1, bool(info.admissible)),
+            bool(info.dense), int(info.rank), float(info.truncation_error),
+            float(info.quantization_bound), int(info.logical_bits), int(info.allocated_bytes))
+
     @property
     def blocks(self) -> list:
-        out = []
-        info = _BlockInfo()
-        for i in range(int(self._info().nblocks)):
-            check(_lib.qh_host_block_info(self._h, i, C.byref(info)))
-            out.append(BlockInfo(
-                Rectangle(info.row0, info.row1, info.col0, info.col1, bool(info.admissible)),
-                bool(info.dense), int(info.rank), float(info.truncation_error),
-                float(info.quantization_bound), int(info.logical_bits),
-                int(info.allocated_bytes)))
-        return out
+        return [self.block(i) for i in range(int(self._info().nblocks))]
 
     @property
     def permutation(self) -> np.ndarray:
@@ -401,7 +401,9 @@ class NativeHMatrix:
         """(words, scales, widths) of one side (0 left, 1 right) of a block."""
         count = _u64()
         check(_lib.qh_host_block_words(self._h, index, side, None, C.byref(count), None, None))
-        rank = self.blocks[index].rank
+        info = _BlockInfo()
+        check(_lib.qh_host_block_info(self._h, index, C.byref(info)))
+        rank = int(info.rank)
         words = np.zeros(count.value, dtype=np.uint32)
         scales = np.zeros(rank, dtype=np.float64)
         widths = np.zeros(rank, dtype=np.int32)

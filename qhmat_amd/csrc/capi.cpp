@@ -32,6 +32,14 @@ struct qh_partition {
     qhamd::PartitionOut out;
 };
 
+struct qh_batch {
+    qhamd::Batch* batch = nullptr;
+};
+
+struct qh_graph {
+    void* exec = nullptr;  // hipGraphExec_t
+};
+
 namespace {
 thread_local std::string g_error;
 
@@ -128,6 +136,85 @@ qh_status qh_apply(const qh_store* store, const double* x, int64_t ldx, int64_t 
         need(store, "store");
         qhamd::apply(*store->dev, x, ldx, k, y, ldy, precision, stream, workspace,
                      workspace_bytes);
+    });
+}
+
+qh_status qh_batch_create(const qh_store* const* stores, const int64_t* k, uint64_t count,
+                          qh_batch** out) {
+    return guarded([&] {
+        need(stores, "stores");
+        need(k, "right-hand-side counts");
+        need(out, "output handle");
+        *out = nullptr;
+        std::vector<const qhamd::DeviceStore*> dev(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            need(stores[i], "store");
+            dev[i] = stores[i]->dev;
+        }
+        auto* b = new qh_batch;
+        try {
+            b->batch = qhamd::create_batch(dev.data(), k, count);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+qh_status qh_batch_destroy(qh_batch* batch) {
+    return guarded([&] {
+        if (!batch) return;
+        qhamd::destroy_batch(batch->batch);
+        delete batch;
+    });
+}
+
+qh_status qh_batch_layout(const qh_batch* batch, uint64_t* offsets, uint64_t* total,
+                          uint64_t* workspace_bytes) {
+    return guarded([&] {
+        need(batch, "batch");
+        const auto& b = *batch->batch;
+        if (offsets)
+            for (std::size_t i = 0; i < b.offset.size(); ++i) offsets[i] = b.offset[i];
+        if (total) *total = b.total;
+        if (workspace_bytes) *workspace_bytes = b.z_total * sizeof(double);
+    });
+}
+
+qh_status qh_batch_apply(const qh_batch* batch, const double* x, double* y, int precision,
+                         void* stream, void* workspace, uint64_t workspace_bytes) {
+    return guarded([&] {
+        need(batch, "batch");
+        qhamd::batch_apply(*batch->batch, x, y, precision, stream, workspace, workspace_bytes);
+    });
+}
+
+qh_status qh_batch_graph_create(const qh_batch* batch, const double* x, double* y,
+                                int precision, void* workspace, uint64_t workspace_bytes,
+                                qh_graph** out) {
+    return guarded([&] {
+        need(batch, "batch");
+        need(out, "output handle");
+        *out = nullptr;
+        void* exec = qhamd::batch_graph(*batch->batch, x, y, precision, workspace,
+                                        workspace_bytes);
+        *out = new qh_graph{exec};
+    });
+}
+
+qh_status qh_graph_launch(const qh_graph* graph, void* stream) {
+    return guarded([&] {
+        need(graph, "graph");
+        qhamd::graph_launch(graph->exec, stream);
+    });
+}
+
+qh_status qh_graph_destroy(qh_graph* graph) {
+    return guarded([&] {
+        if (!graph) return;
+        qhamd::graph_destroy(graph->exec);
+        delete graph;
     });
 }
 
@@ -254,8 +341,7 @@ qh_status qh_host_block_info(const qh_host* h, uint64_t index, qh_block_info* ou
         need(out, "info");
         const auto& blocks = h->store.blocks();
         if (index >= blocks.size()) throw std::invalid_argument("block index out of range");
-        const auto all = h->store.summaries();
-        const auto& s = all[index];
+        const auto s = h->store.summary(index);
         out->row0 = s.rect.row0;
         out->row1 = s.rect.row1;
         out->col0 = s.rect.col0;

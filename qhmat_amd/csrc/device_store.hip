@@ -103,20 +103,24 @@ __device__ __forceinline__ void stage_quads(std::uint32_t* __restrict__ lds,
 
 // ---------------------------------------------------------------------------
 // Pass 1: right factor, one wave per (block, column tile).
-template <int KB, bool FP32>
-__global__ __launch_bounds__(64) void right_factor_pass(
-    const UnitDev* __restrict__ units, const BlockDev* __restrict__ blocks,
-    const std::int32_t* __restrict__ widths, const std::uint32_t* __restrict__ words,
-    const std::int32_t* __restrict__ perm, const double* __restrict__ x, std::int64_t ldx, int k,
-    double* __restrict__ zp) {
-    using Real = typename std::conditional<FP32, float, double>::type;
-    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
-    __shared__ Real xs[kTile][KB];
+template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
 
-    const UnitDev u = units[blockIdx.x];
-    const BlockDev B = blocks[u.block];
+template <int KB, bool FP32>
+__device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint32_t unit, int p0,
+                                                  std::uint32_t* __restrict__ stage,
+                                                  real_t<FP32> (*xs)[KB]) {
+    using Real = real_t<FP32>;
+    const std::int32_t* __restrict__ widths = M.widths;
+    const std::uint32_t* __restrict__ words = M.words;
+    const std::int32_t* __restrict__ perm = M.perm;
+    const double* __restrict__ x = M.x;
+    double* __restrict__ zp = M.zp;
+    const std::int64_t ldx = M.ldx;
+    const int k = M.k;
+
+    const UnitDev u = M.units[unit];
+    const BlockDev B = M.blocks[u.block];
     const int lane = threadIdx.x;
-    const int p0 = blockIdx.y * KB;
     const int kk = min(KB, k - p0);
     const std::uint32_t C = u.cols;
 
@@ -206,24 +210,76 @@ __global__ __launch_bounds__(64) void right_factor_pass(
     }
 }
 
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64) void right_factor_pass(MemberView M) {
+    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
+    __shared__ real_t<FP32> xs[kTile][KB];
+    right_factor_unit<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, stage, xs);
+}
+
+// The member owning global work index `index`: first[m] <= index < first[m+1].
+__device__ __forceinline__ std::uint32_t locate_member(const std::uint32_t* __restrict__ first,
+                                                       std::uint32_t count, std::uint32_t index) {
+    std::uint32_t lo = 0, hi = count;  // first[lo] <= index < first[hi]
+    while (hi - lo > 1) {
+        const std::uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= index) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Batched: the work of every member of one launch, x / y / workspace given
+// as offsets (xoff, yoff, zoff) from the call's base pointers.
+__device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__ members,
+                                                  std::uint32_t m, const double* x, double* y,
+                                                  double* zp) {
+    MemberView M = members[m];
+    M.x = x + reinterpret_cast<std::uintptr_t>(M.x);
+    M.y = y + reinterpret_cast<std::uintptr_t>(M.y);
+    M.zp = zp + reinterpret_cast<std::uintptr_t>(M.zp);
+    return M;
+}
+
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64) void right_factor_batch(const MemberView* __restrict__ members,
+                                                         const std::uint32_t* __restrict__ first,
+                                                         std::uint32_t count, const double* x,
+                                                         double* y, double* zp) {
+    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
+    __shared__ real_t<FP32> xs[kTile][KB];
+    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+    const MemberView M = bind_member(members, m, x, y, zp);
+    const int p0 = blockIdx.y * KB;
+    if (p0 >= M.k) return;
+    right_factor_unit<KB, FP32>(M, blockIdx.x - first[m], p0, stage, xs);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 2: row tiles, kRowWaves waves per tile.
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kRowWaves) void row_tile_pass(
-    const TileDev* __restrict__ tiles, const EntryDev* __restrict__ entries,
-    const BlockDev* __restrict__ blocks, const DenseDev* __restrict__ denses,
-    const std::int32_t* __restrict__ widths, const double* __restrict__ lscale,
-    const double* __restrict__ rscale, const std::uint32_t* __restrict__ words,
-    const double* __restrict__ dvals, const std::int32_t* __restrict__ perm,
-    const double* __restrict__ zp, const double* __restrict__ x, std::int64_t ldx, int k,
-    double* __restrict__ y, std::int64_t ldy) {
-    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
-    __shared__ double partial[kRowWaves - 1][2][64];
+__device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t tile_index,
+                                              int p0,
+                                              std::uint32_t (*stage_all)[kStageWords + 8],
+                                              double (*partial)[2][64]) {
+    const EntryDev* __restrict__ entries = M.entries;
+    const BlockDev* __restrict__ blocks = M.blocks;
+    const DenseDev* __restrict__ denses = M.denses;
+    const std::int32_t* __restrict__ widths = M.widths;
+    const double* __restrict__ lscale = M.lscale;
+    const double* __restrict__ rscale = M.rscale;
+    const std::uint32_t* __restrict__ words = M.words;
+    const double* __restrict__ dvals = M.dense;
+    const std::int32_t* __restrict__ perm = M.perm;
+    const double* __restrict__ zp = M.zp;
+    const double* __restrict__ x = M.x;
+    double* __restrict__ y = M.y;
+    const std::int64_t ldx = M.ldx, ldy = M.ldy;
+    const int k = M.k;
 
-    const TileDev tile = tiles[blockIdx.x];
+    const TileDev tile = M.tiles[tile_index];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int p0 = blockIdx.y * KB;
     const int kk = min(KB, k - p0);
     const std::uint32_t R = tile.rows;
     const bool active = lane < int(R);
@@ -345,6 +401,26 @@ __global__ __launch_bounds__(64 * kRowWaves) void row_tile_pass(
     }
 }
 
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kRowWaves) void row_tile_pass(MemberView M) {
+    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
+    __shared__ double partial[kRowWaves - 1][2][64];
+    row_tile_unit<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, stage_all, partial);
+}
+
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kRowWaves) void row_tile_batch(
+    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
+    std::uint32_t count, const double* x, double* y, double* zp) {
+    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
+    __shared__ double partial[kRowWaves - 1][2][64];
+    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+    const MemberView M = bind_member(members, m, x, y, zp);
+    const int p0 = blockIdx.y * KB;
+    if (p0 >= M.k) return;  // the whole workgroup leaves together
+    row_tile_unit<KB, FP32>(M, blockIdx.x - first[m], p0, stage_all, partial);
+}
+
 // Test hook: codes of one factor side of one block, from its tile chunks,
 // written component-major in the block's own row order.
 __global__ void decode_side(const std::uint32_t* __restrict__ words,
@@ -361,6 +437,31 @@ __global__ void decode_side(const std::uint32_t* __restrict__ words,
     }
 }
 
+MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
+                       std::int64_t ldy, double* zp) {
+    MemberView M{};
+    M.perm = s.perm;
+    M.words = s.words;
+    M.widths = s.widths;
+    M.lscale = s.lscale;
+    M.rscale = s.rscale;
+    M.dense = s.dense;
+    M.blocks = s.blocks;
+    M.denses = s.denses;
+    M.units = s.units;
+    M.tiles = s.tiles;
+    M.entries = s.entries;
+    M.x = x;
+    M.y = y;
+    M.zp = zp;
+    M.ldx = ldx;
+    M.ldy = ldy;
+    M.k = k;
+    M.units_count = s.n_units;
+    M.tiles_count = s.n_tiles;
+    return M;
+}
+
 // Event slots of the next profiled apply, or null when not profiling.
 hipEvent_t* profile_slot(const DeviceStore& s) {
     auto& p = s.profile;
@@ -372,19 +473,46 @@ template <int KB, bool FP32>
 void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
                   std::int64_t ldy, double* zp, hipStream_t stream) {
     const unsigned passes = unsigned((k + KB - 1) / KB);
+    const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
     if (s.n_units) {
         hipLaunchKernelGGL((right_factor_pass<KB, FP32>), dim3(s.n_units, passes), dim3(64), 0,
-                           stream, s.units, s.blocks, s.widths, s.words, s.perm, x, ldx, k, zp);
+                           stream, M);
         hip_check(hipGetLastError(), "right-factor pass launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
     hipLaunchKernelGGL((row_tile_pass<KB, FP32>), dim3(s.n_tiles, passes), dim3(64 * kRowWaves),
-                       0, stream, s.tiles, s.entries, s.blocks, s.denses, s.widths, s.lscale,
-                       s.rscale, s.words, s.dense, s.perm, zp, x, ldx, k, y, ldy);
+                       0, stream, M);
     hip_check(hipGetLastError(), "row-tile pass launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
+}
+
+template <int KB, bool FP32>
+void launch_batch_class(const BatchClass& c, const double* x, double* y, double* zp,
+                        hipStream_t stream) {
+    const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
+    const std::uint32_t count = std::uint32_t(c.member_count);
+    if (c.total_units) {
+        hipLaunchKernelGGL((right_factor_batch<KB, FP32>), dim3(c.total_units, passes), dim3(64),
+                           0, stream, c.members, c.first_unit, count, x, y, zp);
+        hip_check(hipGetLastError(), "batched right-factor pass launch");
+    }
+    hipLaunchKernelGGL((row_tile_batch<KB, FP32>), dim3(c.total_tiles, passes),
+                       dim3(64 * kRowWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
+    hip_check(hipGetLastError(), "batched row-tile pass launch");
+}
+
+template <bool FP32>
+void dispatch_batch_class(const BatchClass& c, const double* x, double* y, double* zp,
+                          hipStream_t st) {
+    switch (c.kb) {
+        case 1: return launch_batch_class<1, FP32>(c, x, y, zp, st);
+        case 2: return launch_batch_class<2, FP32>(c, x, y, zp, st);
+        case 4: return launch_batch_class<4, FP32>(c, x, y, zp, st);
+        case 8: return launch_batch_class<8, FP32>(c, x, y, zp, st);
+        default: return launch_batch_class<16, FP32>(c, x, y, zp, st);
+    }
 }
 
 template <bool FP32>
@@ -884,6 +1012,152 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
     }
     (void)hipFree(dev);
     hip_check(e, "decode launch");
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+
+namespace {
+int rhs_class(std::int64_t k) {
+    return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 16;
+}
+}  // namespace
+
+Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std::uint64_t count) {
+    require(count > 0 && stores && k, "empty batch");
+    auto* b = new Batch;
+    try {
+        b->device = stores[0] ? stores[0]->device : 0;
+        std::uint64_t off = 0, zoff = 0;
+        for (std::uint64_t i = 0; i < count; ++i) {
+            require(stores[i] != nullptr, "null store in batch");
+            require(stores[i]->device == b->device, "batch members must share one device");
+            require(k[i] > 0 && k[i] <= std::int64_t(std::numeric_limits<int>::max()),
+                    "right-hand-side shape mismatch");
+            b->stores.push_back(stores[i]);
+            b->k.push_back(k[i]);
+            b->offset.push_back(off);
+            b->zoffset.push_back(zoff);
+            off += stores[i]->order * std::uint64_t(k[i]);
+            zoff += stores[i]->z_rows * std::uint64_t(k[i]);
+        }
+        b->total = off;
+        b->z_total = zoff;
+        DeviceScope scope(b->device);
+        for (int kb : {1, 2, 4, 8, 16}) {
+            BatchClass c;
+            c.kb = kb;
+            std::vector<MemberView> views;
+            std::vector<std::uint32_t> fu{0}, ft{0};
+            for (std::uint64_t i = 0; i < count; ++i) {
+                if (rhs_class(b->k[i]) != kb) continue;
+                const DeviceStore& s = *b->stores[i];
+                // x, y and the workspace as element offsets from the call's bases
+                views.push_back(member_view(
+                    s, reinterpret_cast<const double*>(std::uintptr_t(b->offset[i])), b->k[i],
+                    int(b->k[i]), reinterpret_cast<double*>(std::uintptr_t(b->offset[i])),
+                    b->k[i], reinterpret_cast<double*>(std::uintptr_t(b->zoffset[i]))));
+                require(std::uint64_t(fu.back()) + s.n_units < (std::uint64_t{1} << 31) &&
+                            std::uint64_t(ft.back()) + s.n_tiles < (std::uint64_t{1} << 31),
+                        "batch too large for one launch");
+                fu.push_back(fu.back() + s.n_units);
+                ft.push_back(ft.back() + s.n_tiles);
+                c.max_k = std::max(c.max_k, b->k[i]);
+            }
+            if (views.empty()) continue;
+            c.member_count = views.size();
+            c.total_units = fu.back();
+            c.total_tiles = ft.back();
+            std::uint64_t bytes = 0;
+            c.members = upload(views, bytes);
+            c.first_unit = upload(fu, bytes);
+            c.first_tile = upload(ft, bytes);
+            b->classes.push_back(c);
+        }
+    } catch (...) {
+        destroy_batch(b);
+        throw;
+    }
+    return b;
+}
+
+void destroy_batch(Batch* b) {
+    if (!b) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(b->device);
+    for (auto& c : b->classes)
+        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_unit),
+                        static_cast<void*>(c.first_tile)})
+            if (p) (void)hipFree(p);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete b;
+}
+
+void batch_apply(const Batch& b, const double* x, double* y, int precision, void* stream,
+                 void* workspace, std::uint64_t workspace_bytes) {
+    require(x != nullptr && y != nullptr, "null right-hand side or result pointer");
+    require(precision == QH_PRECISION_FP64 || precision == QH_PRECISION_FP32SPLIT,
+            "unknown precision mode");
+    const std::uint64_t need = b.z_total * sizeof(double);
+    require(workspace == nullptr || workspace_bytes >= need, "workspace is too small");
+    DeviceScope scope(b.device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double* zp = static_cast<double*>(workspace);
+    bool owned = false;
+    if (!zp && need) {
+        hip_check(hipMallocAsync(reinterpret_cast<void**>(&zp), need, st), "hipMallocAsync");
+        owned = true;
+    }
+    try {
+        for (const auto& c : b.classes) {
+            if (precision == QH_PRECISION_FP64)
+                dispatch_batch_class<false>(c, x, y, zp, st);
+            else
+                dispatch_batch_class<true>(c, x, y, zp, st);
+        }
+    } catch (...) {
+        if (owned) (void)hipFreeAsync(zp, st);
+        throw;
+    }
+    if (owned) hip_check(hipFreeAsync(zp, st), "hipFreeAsync");
+}
+
+void* batch_graph(const Batch& b, const double* x, double* y, int precision, void* workspace,
+                  std::uint64_t workspace_bytes) {
+    require(workspace != nullptr || b.z_total == 0, "graph capture needs a workspace");
+    DeviceScope scope(b.device);
+    hipStream_t st = nullptr;
+    hip_check(hipStreamCreate(&st), "hipStreamCreate");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        try {
+            batch_apply(b, x, y, precision, st, workspace, workspace_bytes);
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        e = hipStreamEndCapture(st, &graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(st);
+    hip_check(e, "graph capture");
+    return exec;
+}
+
+void graph_launch(void* exec, void* stream) {
+    require(exec != nullptr, "null graph");
+    hip_check(hipGraphLaunch(static_cast<hipGraphExec_t>(exec), static_cast<hipStream_t>(stream)),
+              "hipGraphLaunch");
+}
+
+void graph_destroy(void* exec) {
+    if (exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec));
 }
 
 }  // namespace qhamd

@@ -59,6 +59,31 @@ struct DenseDev {
     std::uint64_t off;
 };
 
+// One store's device arrays plus one call's operands: everything a work item
+// of either pass reads.  Passed by value for a single apply; a batch keeps a
+// table of them in device memory with x, y and workspace given as offsets
+// from the call's base pointers.
+struct MemberView {
+    const std::int32_t* perm;
+    const std::uint32_t* words;
+    const std::int32_t* widths;
+    const double* lscale;
+    const double* rscale;
+    const double* dense;
+    const BlockDev* blocks;
+    const DenseDev* denses;
+    const UnitDev* units;
+    const TileDev* tiles;
+    const EntryDev* entries;
+    const double* x;
+    double* y;
+    double* zp;
+    std::int64_t ldx, ldy;
+    std::int32_t k;
+    std::uint32_t units_count, tiles_count;
+    std::uint32_t pad;
+};
+
 // Host record of one chunk (test hook and statistics).
 struct ChunkRef {
     std::uint32_t first, count;  // rows/columns of the block it holds
@@ -125,6 +150,42 @@ Plan build_plan(const qh_store_desc& desc);
 // returns y = H x in FP64.  A check of the layout without a device.
 void emulate_plan(const Plan& plan, const double* x, std::int64_t k, double* y);
 DeviceStore* create_device_store(const qh_store_desc& desc, int device);
+
+// One launch pair of a batch: the members whose right-hand-side count rounds
+// up to the same power of two kb (<= 16; larger counts take several passes).
+struct BatchClass {
+    int kb = 1;
+    std::uint64_t member_count = 0;
+    std::int64_t max_k = 1;
+    std::uint32_t total_units = 0, total_tiles = 0;
+    MemberView* members = nullptr;        // device table; x, y, zp hold offsets
+    std::uint32_t* first_unit = nullptr;  // device, member_count + 1 prefix counts
+    std::uint32_t* first_tile = nullptr;
+};
+
+// A fixed set of resident stores applied together.  Member i reads its
+// order_i x k_i right-hand sides at x + offset[i] (row-major, leading
+// dimension k_i), writes y at the same offset, and uses workspace doubles
+// from zoffset[i].
+struct Batch {
+    int device = 0;
+    std::vector<const DeviceStore*> stores;
+    std::vector<std::int64_t> k;
+    std::vector<std::uint64_t> offset, zoffset;
+    std::uint64_t total = 0, z_total = 0;
+    std::vector<BatchClass> classes;
+};
+
+Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std::uint64_t count);
+void destroy_batch(Batch* b);
+void batch_apply(const Batch& b, const double* x, double* y, int precision, void* stream,
+                 void* workspace, std::uint64_t workspace_bytes);
+// Captures one batch apply on a private stream into an instantiated graph
+// (hipGraphExec_t returned as void*); the workspace must be given.
+void* batch_graph(const Batch& b, const double* x, double* y, int precision, void* workspace,
+                  std::uint64_t workspace_bytes);
+void graph_launch(void* exec, void* stream);
+void graph_destroy(void* exec);
 void destroy_device_store(DeviceStore* s);
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k);
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,

@@ -487,7 +487,13 @@ void HostStore::reconstruct(double* out) const {
 std::vector<BlockSummary> HostStore::summaries() const {
     std::vector<BlockSummary> out;
     out.reserve(blocks_.size());
-    for (const auto& b : blocks_) {
+    for (std::size_t i = 0; i < blocks_.size(); ++i) out.push_back(summary(i));
+    return out;
+}
+
+BlockSummary HostStore::summary(std::size_t index) const {
+    const HostBlock& b = blocks_.at(index);
+    {
         BlockSummary s{b.rect, b.is_dense(), b.widths.size(), b.truncation_error,
                        b.quantization_bound, 0, 0};
         if (b.is_dense()) {
@@ -509,9 +515,8 @@ std::vector<BlockSummary> HostStore::summaries() const {
             bytes = add_checked(bytes, mul_checked(b.widths.size(), sizeof(int)));
             s.allocated_bytes = bytes;
         }
-        out.push_back(s);
+        return s;
     }
-    return out;
 }
 
 std::uint64_t HostStore::logical_bits() const {

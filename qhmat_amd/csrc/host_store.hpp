@@ -88,6 +88,7 @@ public:
     void reconstruct(double* out) const;
 
     std::vector<BlockSummary> summaries() const;
+    BlockSummary summary(std::size_t index) const;
     std::uint64_t logical_bits() const;
     std::uint64_t allocated_bytes() const;
     std::uint64_t resident_bytes() const;

@@ -124,7 +124,7 @@ def decode_codes(native, block: int, side: int) -> np.ndarray:
     """Codes of one block side, (rank, rows) int32, decoded by the device."""
     require_device()
     torch = _torch()
-    info = native.blocks[block]
+    info = native.block(block)
     if info.dense:
         raise ValueError("dense blocks carry no codes")
     r = info.rectangle

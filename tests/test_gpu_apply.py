@@ -57,7 +57,7 @@ NAMES = ["two_by_two"] + [f"orthogonal_b{b}" for b in (2, 4, 8, 12, 16, 24, 32)]
 
 
 def _check_codes(native):
-    for index, info in enumerate(native.blocks):
+    for index, info in enumerate(native.blocks):  # one pass over the metadata
         if info.dense or info.rank == 0:
             continue
         r = info.rectangle
