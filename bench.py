@@ -240,7 +240,7 @@ def main() -> int:
             "sample": f"{base['runs']} applies of the same store at k=1 (median "
                       f"{base['seconds'] * 1e3:.1f} ms/apply)"}
     for key in ("fp32split_k1", "fp64_k1", "fp32split_k32", "fp64_k32"):
-        line[key] = {a: (round(v, 6) if isinstance(v, float) else v)
+        line[key] = {a: (float(f"{v:.6g}") if isinstance(v, float) else v)
                      for a, v in line[key].items()}
     print(json.dumps(line), flush=True)
     return 0

@@ -84,6 +84,17 @@ __host__ __device__ __forceinline__ std::uint32_t stream_words(std::uint32_t cou
     return (count * bits + 31u) >> 5;
 }
 
+// Lane j's value, broadcast to the wave (v_readlane_b32; j is wave-uniform).
+__device__ __forceinline__ float lane_value(float v, std::uint32_t j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_value(double v, std::uint32_t j) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), j));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), j));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // LDS written by some lanes of a wave and read by others: keep the compiler
 // from moving accesses across this point and wait for the LDS queue.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -92,13 +103,26 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Copies `quads` 16-byte groups from global (16-byte aligned) to LDS.
+// Copies `quads` 16-byte groups from global (16-byte aligned) to LDS.  Up to
+// four loads per lane are in flight before the first LDS write waits.
 __device__ __forceinline__ void stage_quads(std::uint32_t* __restrict__ lds,
                                             const std::uint32_t* __restrict__ src,
                                             std::uint32_t quads, int lane) {
     const uint4* s = reinterpret_cast<const uint4*>(src);
     uint4* d = reinterpret_cast<uint4*>(lds);
-    for (std::uint32_t q = lane; q < quads; q += 64) d[q] = s[q];
+    for (std::uint32_t q0 = 0; q0 < quads; q0 += 256) {
+        uint4 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const std::uint32_t q = q0 + lane + 64 * i;
+            if (q < quads) v[i] = s[q];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const std::uint32_t q = q0 + lane + 64 * i;
+            if (q < quads) d[q] = v[i];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -124,10 +148,18 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
     const int kk = min(KB, k - p0);
     const std::uint32_t C = u.cols;
 
+    // Request everything before waiting on anything: the first pass's widths,
+    // the permutation rows, the whole chunk when it fits the stage buffer,
+    // then the x rows.
+    const bool whole = u.words + 5 <= std::uint32_t(kStageWords);
+    const std::uint32_t delta0 = std::uint32_t(u.chunk & 3u);
+    const std::uint32_t jn0 = min(std::uint32_t(kPassComponents), B.rank);
+    std::uint32_t wl = lane < int(jn0) ? std::uint32_t(widths[B.comp_base + lane]) : 0u;
+    const std::int64_t xrow = lane < int(C) ? std::int64_t(perm[u.col0 + lane]) * ldx + p0 : 0;
+    if (whole) stage_quads(stage, words + (u.chunk - delta0), (u.words + delta0 + 1 + 3) >> 2, lane);
     if (lane < int(C)) {
-        const double* xr = x + std::int64_t(perm[u.col0 + lane]) * ldx + p0;
 #pragma unroll
-        for (int p = 0; p < KB; ++p) xs[lane][p] = p < kk ? Real(xr[p]) : Real(0);
+        for (int p = 0; p < KB; ++p) xs[lane][p] = p < kk ? Real(x[xrow + p]) : Real(0);
     }
 
     const int g = lane >> 2;  // component slot of this lane in a pass
@@ -136,7 +168,7 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
     for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
         const std::uint32_t jn = min(std::uint32_t(kPassComponents), B.rank - j0);
         // widths and stream offsets of this pass's components (prefix scan)
-        const std::uint32_t wl = lane < int(jn) ? std::uint32_t(widths[B.comp_base + j0 + lane]) : 0u;
+        if (j0 > 0) wl = lane < int(jn) ? std::uint32_t(widths[B.comp_base + j0 + lane]) : 0u;
         const std::uint32_t nwl = stream_words(C, wl);
         std::uint32_t incl = nwl;
 #pragma unroll
@@ -148,14 +180,17 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
         const std::uint32_t bits = __shfl(wl, g & (kPassComponents - 1), 64);
         const std::uint32_t seg = __shfl(incl - nwl, g & (kPassComponents - 1), 64);
 
-        const std::uint64_t first = u.chunk + done;
-        const std::uint32_t delta = std::uint32_t(first & 3u);
-        stage_quads(stage, words + (first - delta), (total + delta + 1 + 3) >> 2, lane);
+        std::uint32_t base = delta0 + done;  // first word of this pass in the stage buffer
+        if (!whole) {
+            const std::uint64_t first = u.chunk + done;
+            base = std::uint32_t(first & 3u);
+            stage_quads(stage, words + (first - base), (total + base + 1 + 3) >> 2, lane);
+        }
         wave_lds_sync();
 
         double tot[KB];
         const bool mine = g < int(jn);
-        const std::uint32_t* sw = stage + delta + seg;
+        const std::uint32_t* sw = stage + base + seg;
         if (!FP32 || bits <= 24) {
             Real part[KB];
 #pragma unroll
@@ -206,7 +241,7 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
                 if (p < kk) out[p] = tot[p];
         }
         done += total;
-        wave_lds_sync();  // the next pass restages
+        if (!whole) wave_lds_sync();  // the next pass restages
     }
 }
 
@@ -320,57 +355,80 @@ __device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t
         float blk[KB];
 #pragma unroll
         for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
+        // Lanes own components: each loads its width, scales and right-factor
+        // sums at once; the component loop then reads them with readlane.
+        // The whole chunk is staged in the same breath when it fits.
+        const bool whole = ent.words + 5 <= std::uint32_t(kStageWords) && B.rank <= 64;
         std::uint64_t first = ent.chunk;
+        std::uint32_t base = std::uint32_t(first & 3u);
         for (std::uint32_t j0 = 0; j0 < B.rank;) {
-            // the components [j0, j1) whose streams fit in the stage buffer
-            std::uint32_t j1 = j0, total = 0;
-            while (j1 < B.rank) {
-                const std::uint32_t w = stream_words(R, widths[B.comp_base + j1]);
-                if (total + w > kStageWords - 4) break;
-                total += w;
-                ++j1;
-            }
-            const std::uint32_t delta = std::uint32_t(first & 3u);
-            stage_quads(stage, words + (first - delta), (total + delta + 1 + 3) >> 2, lane);
-            wave_lds_sync();
-            std::uint32_t off = delta;
-            for (std::uint32_t j = j0; j < j1; ++j) {
-                const std::uint32_t cj = B.comp_base + j;
-                const std::uint32_t bits = widths[cj];
-                const double s = lscale[cj] * rscale[cj];
-                double z[KB];
+            const std::uint32_t jn = min(64u, B.rank - j0);
+            const bool mine = lane < int(jn);
+            const std::uint32_t cj = B.comp_base + j0 + lane;
+            const std::uint32_t bits_l = mine ? std::uint32_t(widths[cj]) : 0u;
+            const double s_l = mine ? lscale[cj] * rscale[cj] : 0.0;
+            double z_l[KB];
 #pragma unroll
-                for (int p = 0; p < KB; ++p) z[p] = 0.0;
+            for (int p = 0; p < KB; ++p) z_l[p] = 0.0;
+            if (mine) {
                 for (std::uint32_t sg = 0; sg < B.nseg; ++sg) {
-                    const double* zr =
-                        zp + (std::uint64_t(B.zbase) + sg * B.rank + j) * std::uint64_t(k) + p0;
+                    const double* zr = zp + (std::uint64_t(B.zbase) + sg * B.rank + j0 + lane) *
+                                                std::uint64_t(k) + p0;
 #pragma unroll
                     for (int p = 0; p < KB; ++p)
-                        if (p < kk) z[p] += zr[p];
+                        if (p < kk) z_l[p] += zr[p];
                 }
-                const int code = active ? decode_code(stage + off, lane, bits) : 0;
+            }
+            if (whole) stage_quads(stage, words + (first - base), (ent.words + base + 1 + 3) >> 2,
+                                   lane);
+            const std::uint32_t nw_l = mine ? stream_words(R, bits_l) : 0u;
+            std::uint32_t incl = nw_l;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const std::uint32_t t = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += t;
+            }
+            std::uint32_t take = jn;
+            if (!whole) {  // the components whose streams fit the stage buffer
+                const bool fits = mine && incl + base + 5 <= std::uint32_t(kStageWords);
+                take = std::uint32_t(__popcll(__ballot(fits)));
+            }
+            const std::uint32_t total = __builtin_amdgcn_readlane(incl, take - 1);
+            if (!whole)
+                stage_quads(stage, words + (first - base), (total + base + 1 + 3) >> 2, lane);
+            const std::uint32_t off_l = incl - nw_l;
+            real_t<FP32> zs_l[KB];
+#pragma unroll
+            for (int p = 0; p < KB; ++p) zs_l[p] = real_t<FP32>(s_l * z_l[p]);
+            wave_lds_sync();
+            for (std::uint32_t j = 0; j < take; ++j) {
+                const std::uint32_t bits = __builtin_amdgcn_readlane(bits_l, j);
+                const std::uint32_t off = __builtin_amdgcn_readlane(off_l, j);
+                const int code = active ? decode_code(stage + base + off, lane, bits) : 0;
                 if (!FP32) {
                     const double cd = double(code);
 #pragma unroll
-                    for (int p = 0; p < KB; ++p) acc[p] = fma(cd, s * z[p], acc[p]);
+                    for (int p = 0; p < KB; ++p)
+                        acc[p] = fma(cd, double(lane_value(zs_l[p], j)), acc[p]);
                 } else if (bits <= 24) {
                     const float cf = float(code);
 #pragma unroll
-                    for (int p = 0; p < KB; ++p) blk[p] = fmaf(cf, float(s * z[p]), blk[p]);
+                    for (int p = 0; p < KB; ++p)
+                        blk[p] = fmaf(cf, float(lane_value(zs_l[p], j)), blk[p]);
                 } else {
                     const float h = float(code >> 16), l = float(code & 0xFFFF);
 #pragma unroll
                     for (int p = 0; p < KB; ++p) {
-                        const float zf = float(s * z[p]);
+                        const float zf = float(lane_value(zs_l[p], j));
                         blk[p] = fmaf(h, zf * 65536.0f, blk[p]);
                         blk[p] = fmaf(l, zf, blk[p]);
                     }
                 }
-                off += stream_words(R, bits);
             }
             first += total;
-            j0 = j1;
-            wave_lds_sync();  // the next batch restages
+            j0 += take;
+            wave_lds_sync();  // the next round (or entry) restages
+            base = std::uint32_t(first & 3u);
         }
         if (FP32) {
 #pragma unroll
@@ -643,7 +701,8 @@ Plan build_plan(const qh_store_desc& d) {
             for (std::uint64_t i = 0; i < m; ++i)  // row-major in, column-major out
                 for (std::uint64_t c = 0; c < nc; ++c) dvals[at + c * m + i] = v[i * nc + c];
             for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t) {
-                per_tile[t].push_back({kDenseBit | di, std::uint32_t(row_tiles[t].first - r0), 0});
+                per_tile[t].push_back(
+                    {kDenseBit | di, std::uint32_t(row_tiles[t].first - r0), 0u, 0u, 0u});
                 per_cost[t].push_back(3 * nc);
             }
             dn_flops += 2 * m * nc;
@@ -686,7 +745,8 @@ Plan build_plan(const qh_store_desc& d) {
             const std::uint64_t chunk_words = words.size() - at;
             pad4();
             s->left_chunks.back().push_back({std::uint32_t(a), std::uint32_t(rows), at});
-            per_tile[t].push_back({li, std::uint32_t(a), at});
+            require(chunk_words + 8 < (std::uint64_t{1} << 32), "block chunk too large");
+            per_tile[t].push_back({li, std::uint32_t(a), std::uint32_t(chunk_words), 0u, at});
             per_cost[t].push_back(8 * r + chunk_words / 16);
         }
         // right factor: one chunk (and one work unit) per column tile
@@ -698,10 +758,13 @@ Plan build_plan(const qh_store_desc& d) {
             const std::uint64_t at = words.size();
             for (std::uint64_t j = 0; j < r; ++j)
                 append_codes(d.rwords[b] + roff[j], rcnt[j], a, cols, d.widths[b][j], words);
+            const std::uint64_t chunk_words = words.size() - at;
+            require(chunk_words + 8 < (std::uint64_t{1} << 32), "block chunk too large");
             pad4();
             s->right_chunks.back().push_back({std::uint32_t(a), std::uint32_t(cols), at});
             units.push_back({li, std::uint32_t(col_tiles[t].first), std::uint32_t(cols),
-                             std::uint32_t(z_rows + B.nseg * r), at});
+                             std::uint32_t(z_rows + B.nseg * r), std::uint32_t(chunk_words), 0u,
+                             at});
             ++B.nseg;
         }
         z_rows += std::uint64_t(B.nseg) * r;
@@ -842,6 +905,12 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
     for (const UnitDev& u : pl.units) {
         const BlockDev& B = at(pl.blocks, u.block);
         if (u.cols == 0 || u.cols > std::uint32_t(kTile)) throw std::out_of_range("unit width");
+        std::uint64_t all = 0;
+        for (std::uint32_t j = 0; j < B.rank; ++j)
+            all += stream_words(u.cols, at(pl.widths, B.comp_base + j));
+        if (all != u.words) throw std::out_of_range("unit word count");
+        if (u.words + 5 <= std::uint32_t(kStageWords))  // staged whole, one word past the end
+            at(pl.words, (u.chunk & ~std::uint64_t(3)) + 4 * ((u.words + (u.chunk & 3) + 4) >> 2) - 1);
         std::uint64_t done = 0;
         for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
             const std::uint32_t jn = std::min<std::uint32_t>(kPassComponents, B.rank - j0);
@@ -886,6 +955,12 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
                             continue;
                         }
                         const BlockDev& B = at(pl.blocks, ent.what);
+                        std::uint64_t all = 0;
+                        for (std::uint32_t j = 0; j < B.rank; ++j)
+                            all += stream_words(T.rows, at(pl.widths, B.comp_base + j));
+                        if (all != ent.words) throw std::out_of_range("entry word count");
+                        at(pl.words, (ent.chunk & ~std::uint64_t(3)) +
+                                         4 * ((ent.words + (ent.chunk & 3) + 4) >> 2) - 1);
                         std::uint64_t off = ent.chunk;
                         for (std::uint32_t j = 0; j < B.rank; ++j) {
                             const std::uint32_t cj = B.comp_base + j;

@@ -20,7 +20,7 @@
 namespace qhamd {
 
 constexpr int kTile = 64;          // rows per row tile, columns per column tile
-constexpr int kRowWaves = 8;       // waves per row-tile workgroup
+constexpr int kRowWaves = 16;      // waves per row-tile workgroup
 constexpr int kStageWords = 1024;  // LDS words a wave stages at a time
 constexpr int kPassComponents = 16;  // components per right-factor pass (4 lanes each)
 
@@ -34,14 +34,16 @@ struct BlockDev {
 // Right-factor unit: block `block` restricted to columns [col0, col0+cols).
 struct UnitDev {
     std::uint32_t block, col0, cols, zrow;
-    std::uint64_t chunk;  // first word, multiple of 4
+    std::uint32_t words, pad;  // words of the chunk (all components, unpadded)
+    std::uint64_t chunk;       // first word, multiple of 4
 };
 
 // Row-tile entry: kDenseBit | dense index, or low-rank block index.
 struct EntryDev {
     std::uint32_t what;
-    std::uint32_t row;    // dense: first tile row relative to the block
-    std::uint64_t chunk;  // low-rank: first word of the block's chunk for this tile
+    std::uint32_t row;         // dense: first tile row relative to the block
+    std::uint32_t words, pad;  // low-rank: words of the chunk (all components)
+    std::uint64_t chunk;       // low-rank: first word of the block's chunk for this tile
 };
 constexpr std::uint32_t kDenseBit = 0x80000000u;
 

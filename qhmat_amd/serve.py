@@ -1,0 +1,277 @@
+"""Serving loop: resident H-matrices, HBM-budget admission, continuous
+batching of apply requests, p50/p99 latency.
+
+* Stores are admitted against an HBM budget (their resident device bytes);
+  a store that does not fit raises AdmissionError.
+* Requests (store name, FP64 right-hand sides n x k) are queued.  The
+  dispatcher takes whatever is queued when a pass slot is free, merges the
+  requests that target the same store into one k-wide apply (requests whose
+  operands would overflow the budget wait for memory to free up), and
+  launches it on one of several streams, so passes overlap on the device and
+  requests that arrive meanwhile join the next pass.
+* A completion thread waits for each pass and hands every request its own
+  columns of the result.
+
+A right-hand-side column's arithmetic does not depend on which other columns
+share its pass, so a request's result is bitwise the same alone and in any
+mix (tests/test_gpu_serve.py).
+"""
+
+from __future__ import annotations
+
+import collections
+import queue
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .device import apply as device_apply
+from .device import workspace_bytes
+
+
+class AdmissionError(RuntimeError):
+    """The store does not fit in the serving HBM budget."""
+
+
+@dataclass
+class _Request:
+    store: str
+    x: object  # torch tensor (n, k) on the device
+    ready: object  # torch.cuda.Event recorded after x was written
+    k: int
+    arrival: float
+    future: Future = field(default_factory=Future)
+
+
+@dataclass
+class _Resident:
+    native: object
+    order: int
+    device_bytes: int
+
+
+class Server:
+    """Continuous-batching apply server on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, precision: str = "fp64",
+                 slots: int = 3, max_rhs: int = 256):
+        import torch
+
+        self.torch = torch
+        self.device = device
+        self.budget = int(hbm_budget)
+        self.precision = precision
+        self.max_rhs = int(max_rhs)
+        self._stores: dict[str, _Resident] = {}
+        self._resident_bytes = 0
+        self._inflight_bytes = 0
+        self._queue: collections.deque[_Request] = collections.deque()
+        self._cv = threading.Condition()
+        self._slots = threading.Semaphore(slots)
+        self._streams = [torch.cuda.Stream(device=device) for _ in range(slots)]
+        self._next_stream = 0
+        self._done: queue.Queue = queue.Queue()
+        self._running = False
+        self.latencies: list[float] = []
+        self.passes = 0
+        self.completed = 0
+        self.completed_rhs = 0
+
+    # -- resident stores ----------------------------------------------------------------
+    def admit(self, name: str, native) -> int:
+        """Uploads `native` and keeps it resident; returns its device bytes."""
+        with self._cv:
+            if name in self._stores:
+                raise ValueError(f"store {name!r} is already resident")
+        native.device_store(self.device)
+        size = int(native.device_stats(self.device)["device_bytes"])
+        with self._cv:
+            if self._resident_bytes + self._inflight_bytes + size > self.budget:
+                native.release_device(self.device)
+                raise AdmissionError(f"store {name!r} ({size} bytes) exceeds the HBM budget")
+            self._stores[name] = _Resident(native, native.order, size)
+            self._resident_bytes += size
+        return size
+
+    def evict(self, name: str) -> None:
+        with self._cv:
+            res = self._stores.pop(name)
+            self._resident_bytes -= res.device_bytes
+        res.native.release_device(self.device)
+
+    @property
+    def resident_bytes(self) -> int:
+        return self._resident_bytes
+
+    # -- requests -------------------------------------------------------------------------
+    def submit(self, name: str, x) -> Future:
+        """Queues y = H_name x; the future resolves to a device tensor (n, k)."""
+        torch = self.torch
+        res = self._stores.get(name)
+        if res is None:
+            raise KeyError(f"store {name!r} is not resident")
+        if isinstance(x, np.ndarray):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64))
+        x = x.to(device=f"cuda:{self.device}", dtype=torch.float64)
+        x = x.reshape(res.order, -1)
+        if not bool(torch.isfinite(x).all()):
+            raise ValueError("nonfinite right-hand side")
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.device))
+        req = _Request(name, x, ready, int(x.shape[1]), time.perf_counter())
+        with self._cv:
+            self._queue.append(req)
+            self._cv.notify()
+        return req.future
+
+    # -- loop -------------------------------------------------------------------------------
+    def start(self) -> "Server":
+        self._running = True
+        self._threads = [threading.Thread(target=self._dispatch_loop, daemon=True),
+                         threading.Thread(target=self._complete_loop, daemon=True)]
+        for t in self._threads:
+            t.start()
+        return self
+
+    def stop(self) -> None:
+        with self._cv:
+            self._running = False
+            self._cv.notify_all()
+        self._threads[0].join()
+        self._done.put(None)
+        self._threads[1].join()
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc):
+        self.stop()
+        return False
+
+    def _take(self) -> dict[str, list[_Request]]:
+        """Everything queued that fits the memory budget, grouped by store."""
+        groups: dict[str, list[_Request]] = collections.OrderedDict()
+        rhs: dict[str, int] = {}
+        kept = collections.deque()
+        while self._queue:
+            req = self._queue.popleft()
+            res = self._stores[req.store]
+            k_now = rhs.get(req.store, 0)
+            cost = 16 * res.order * req.k
+            over_budget = (self._resident_bytes + self._inflight_bytes + cost > self.budget
+                           and (groups or self._inflight_bytes))
+            if k_now + req.k > self.max_rhs or over_budget:
+                kept.append(req)  # waits for the next pass
+                continue
+            groups.setdefault(req.store, []).append(req)
+            rhs[req.store] = k_now + req.k
+            self._inflight_bytes += cost
+        self._queue = kept
+        return groups
+
+    def _dispatch_loop(self) -> None:
+        torch = self.torch
+        while True:
+            self._slots.acquire()
+            with self._cv:
+                while self._running and not self._queue:
+                    self._cv.wait()
+                if not self._running and not self._queue:
+                    self._slots.release()
+                    return
+                groups = self._take()
+            if not groups:
+                self._slots.release()
+                time.sleep(0)
+                continue
+            stream = self._streams[self._next_stream]
+            self._next_stream = (self._next_stream + 1) % len(self._streams)
+            work = []
+            with torch.cuda.stream(stream):
+                for name, reqs in groups.items():
+                    res = self._stores[name]
+                    for r in reqs:
+                        stream.wait_event(r.ready)
+                    x = reqs[0].x if len(reqs) == 1 else torch.cat([r.x for r in reqs], dim=1)
+                    k = x.shape[1]
+                    ws = torch.empty(max(workspace_bytes(res.native, k, self.device), 8),
+                                     dtype=torch.uint8, device=x.device)
+                    y = device_apply(res.native, x, precision=self.precision, stream=stream,
+                                     workspace=ws)
+                    work.append((reqs, y, 16 * res.order * k))
+                done = torch.cuda.Event()
+                done.record(stream)
+            self.passes += 1
+            self._done.put((done, work))
+
+    def _complete_loop(self) -> None:
+        while True:
+            item = self._done.get()
+            if item is None:
+                return
+            done, work = item
+            done.synchronize()
+            now = time.perf_counter()
+            freed = 0
+            for reqs, y, cost in work:
+                col = 0
+                for r in reqs:
+                    r.future.set_result(y[:, col:col + r.k])
+                    col += r.k
+                    self.latencies.append(now - r.arrival)
+                    self.completed += 1
+                    self.completed_rhs += r.k
+                freed += cost
+            with self._cv:
+                self._inflight_bytes -= freed
+                self._cv.notify()
+            self._slots.release()
+
+    # -- statistics ---------------------------------------------------------------------------
+    def latency_ms(self) -> dict:
+        lat = np.asarray(self.latencies) * 1e3
+        if lat.size == 0:
+            return {"p50": None, "p99": None, "mean": None}
+        return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                "mean": float(lat.mean())}
+
+
+def poisson_load(server: Server, names: list[str], orders: dict[str, int], rate: float,
+                 duration: float, ks=(1, 4, 16), session: int = 50, seed: int = 0) -> dict:
+    """Open-loop Poisson arrivals at `rate` req/s for `duration` seconds.
+
+    Requests come in sessions of `session` consecutive requests (a GMRES
+    client's iterations) that target one store with one k.  Returns
+    throughput and latency figures.
+    """
+    import torch
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 1.2) + 10)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < duration]
+    # pre-made right-hand sides per (store, k), already on the device
+    xs = {(nm, k): torch.randn(orders[nm], k, dtype=torch.float64, device=f"cuda:{server.device}")
+          for nm in names for k in ks}
+    torch.cuda.synchronize()
+    futures = []
+    t0 = time.perf_counter()
+    s_name, s_k = None, None
+    for i, at in enumerate(arrivals):
+        if i % session == 0:
+            s_name = names[int(rng.integers(len(names)))]
+            s_k = int(ks[int(rng.integers(len(ks)))])
+        while time.perf_counter() - t0 < at:
+            pass
+        futures.append(server.submit(s_name, xs[(s_name, s_k)]))
+    for f in futures:
+        f.result()
+    elapsed = time.perf_counter() - t0
+    out = {"offered_rate": rate, "requests": len(futures), "seconds": elapsed,
+           "throughput_rps": len(futures) / elapsed,
+           "passes": server.passes}
+    out.update({f"latency_{k}_ms": v for k, v in server.latency_ms().items()})
+    return out

@@ -1,0 +1,82 @@
+"""Serving loop: admission against the HBM budget, continuous batching, and
+each request's result bitwise the same alone and in a mix."""
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def natives():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("gpu tests need a HIP device")
+    return {
+        "square": H.ragged_square(n=900, leaf=30).native,
+        "line": H.line_kernel(200, 12, 1e-6, widths=(4, 8, 12, 16, 24, 32))[0].native,
+        "user": H.user_partition()[0].native,
+    }
+
+
+def test_admission_budget(natives):
+    from qhmat_amd.serve import AdmissionError, Server
+
+    server = Server(hbm_budget=1)
+    with pytest.raises(AdmissionError):
+        server.admit("square", natives["square"])
+    big = Server(hbm_budget=1 << 30)
+    size = big.admit("square", natives["square"])
+    assert size > 0 and big.resident_bytes == size
+    big.evict("square")
+    assert big.resident_bytes == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32split"])
+def test_results_same_alone_and_in_mix(natives, precision):
+    import torch
+
+    from qhmat_amd.serve import Server
+
+    rng = np.random.default_rng(3)
+    plan = [(name, k) for _ in range(6) for name in natives for k in (1, 3, 4)]
+    xs = [torch.from_numpy(rng.normal(size=(natives[n].order, k))).cuda() for n, k in plan]
+    alone = []
+    for (name, _), x in zip(plan, xs):
+        alone.append(natives[name].apply_device(x, precision=precision))
+    torch.cuda.synchronize()
+    with Server(hbm_budget=1 << 32, precision=precision, slots=3) as server:
+        for name, native in natives.items():
+            server.admit(name, native)
+        futures = [server.submit(name, x) for (name, _), x in zip(plan, xs)]
+        results = [f.result(timeout=60) for f in futures]
+        torch.cuda.synchronize()
+        assert server.passes < len(plan)  # requests were merged into shared passes
+    for got, want in zip(results, alone):
+        assert torch.equal(got, want)
+    lat = server.latency_ms()
+    assert lat["p50"] is not None and lat["p99"] >= lat["p50"]
+
+
+def test_requests_wait_for_memory(natives):
+    """A budget that fits the stores but only one request's operands at a
+    time still serves everything (requests queue for memory)."""
+    import torch
+
+    from qhmat_amd.serve import Server
+
+    native = natives["square"]
+    x = torch.randn(native.order, 4, dtype=torch.float64, device="cuda")
+    want = native.apply_device(x)
+    torch.cuda.synchronize()
+    server = Server(hbm_budget=1 << 62)
+    size = server.admit("square", native)
+    server.budget = size + 16 * native.order * 4 + 1  # room for one request in flight
+    with server:
+        futures = [server.submit("square", x) for _ in range(5)]
+        for f in futures:
+            assert torch.equal(f.result(timeout=60), want)
+        assert server.passes == 5
