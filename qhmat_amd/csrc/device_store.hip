@@ -144,7 +144,7 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
 
     const UnitDev u = M.units[unit];
     const BlockDev B = M.blocks[u.block];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int kk = min(KB, k - p0);
     const std::uint32_t C = u.cols;
 
@@ -245,11 +245,16 @@ __device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint
     }
 }
 
+// kUnitWaves units per workgroup, one per wave (a one-wave workgroup would
+// leave the per-CU workgroup limit, not the waves, bounding occupancy).
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64) void right_factor_pass(MemberView M) {
-    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
-    __shared__ real_t<FP32> xs[kTile][KB];
-    right_factor_unit<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, stage, xs);
+__global__ __launch_bounds__(64 * kUnitWaves) void right_factor_pass(MemberView M) {
+    __shared__ __align__(16) std::uint32_t stage[kUnitWaves][kStageWords + 8];
+    __shared__ real_t<FP32> xs[kUnitWaves][kTile][KB];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const std::uint32_t unit = blockIdx.x * kUnitWaves + wave;
+    if (unit >= M.units_count) return;
+    right_factor_unit<KB, FP32>(M, unit, blockIdx.y * KB, stage[wave], xs[wave]);
 }
 
 // The member owning global work index `index`: first[m] <= index < first[m+1].
@@ -277,17 +282,19 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
 }
 
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64) void right_factor_batch(const MemberView* __restrict__ members,
-                                                         const std::uint32_t* __restrict__ first,
-                                                         std::uint32_t count, const double* x,
-                                                         double* y, double* zp) {
-    __shared__ __align__(16) std::uint32_t stage[kStageWords + 8];
-    __shared__ real_t<FP32> xs[kTile][KB];
-    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+__global__ __launch_bounds__(64 * kUnitWaves) void right_factor_batch(
+    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
+    std::uint32_t count, std::uint32_t total, const double* x, double* y, double* zp) {
+    __shared__ __align__(16) std::uint32_t stage[kUnitWaves][kStageWords + 8];
+    __shared__ real_t<FP32> xs[kUnitWaves][kTile][KB];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const std::uint32_t unit = blockIdx.x * kUnitWaves + wave;
+    if (unit >= total) return;
+    const std::uint32_t m = locate_member(first, count, unit);
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;
-    right_factor_unit<KB, FP32>(M, blockIdx.x - first[m], p0, stage, xs);
+    right_factor_unit<KB, FP32>(M, unit - first[m], p0, stage[wave], xs[wave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -328,27 +335,31 @@ __device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t
     for (std::uint32_t e = tile.split[wave]; e < e_end; ++e) {
         const EntryDev ent = entries[e];
         if (ent.what & kDenseBit) {
-            // Diagonal block: D column-major, lanes read one column per step
-            // (coalesced); x rows staged in LDS, read as broadcasts.
+            // Up to kDenseCols columns of a diagonal block (column-major, one
+            // coalesced column per lane-row step): every load is issued before
+            // the first FMA; the x rows are staged in LDS and read as broadcasts.
             const DenseDev D = denses[ent.what & ~kDenseBit];
+            const std::uint32_t cbeg = ent.words, cnt = ent.cols;
             double* xsd = reinterpret_cast<double*>(stage);
-            constexpr std::uint32_t kCols = (kStageWords / 2) / KB < 64 ? (kStageWords / 2) / KB : 64;
-            const double* col = dvals + D.off + ent.row + lane;
-            for (std::uint32_t c0 = 0; c0 < D.n; c0 += kCols) {
-                const std::uint32_t cn = min(kCols, D.n - c0);
-                if (lane < int(cn)) {
-                    const double* xr = x + std::int64_t(perm[D.col0 + c0 + lane]) * ldx + p0;
+            const double* col = dvals + D.off + std::uint64_t(cbeg) * D.m + ent.row + lane;
+            double dv[kDenseCols];
 #pragma unroll
-                    for (int p = 0; p < KB; ++p) xsd[lane * KB + p] = p < kk ? xr[p] : 0.0;
-                }
-                wave_lds_sync();
-                for (std::uint32_t c = 0; c < cn; ++c) {
-                    const double d = active ? col[std::uint64_t(c0 + c) * D.m] : 0.0;
+            for (int i = 0; i < kDenseCols; ++i)
+                dv[i] = (active && i < int(cnt)) ? col[std::uint64_t(i) * D.m] : 0.0;
+            if (lane < int(cnt)) {
+                const double* xr = x + std::int64_t(perm[D.col0 + cbeg + lane]) * ldx + p0;
 #pragma unroll
-                    for (int p = 0; p < KB; ++p) acc[p] = fma(d, xsd[c * KB + p], acc[p]);
-                }
-                wave_lds_sync();
+                for (int p = 0; p < KB; ++p) xsd[lane * KB + p] = p < kk ? xr[p] : 0.0;
             }
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < kDenseCols; ++i) {
+                if (i < int(cnt)) {
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) acc[p] = fma(dv[i], xsd[i * KB + p], acc[p]);
+                }
+            }
+            wave_lds_sync();
             continue;
         }
         const BlockDev B = blocks[ent.what];
@@ -535,8 +546,9 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
     if (s.n_units) {
-        hipLaunchKernelGGL((right_factor_pass<KB, FP32>), dim3(s.n_units, passes), dim3(64), 0,
-                           stream, M);
+        hipLaunchKernelGGL((right_factor_pass<KB, FP32>),
+                           dim3((s.n_units + kUnitWaves - 1) / kUnitWaves, passes),
+                           dim3(64 * kUnitWaves), 0, stream, M);
         hip_check(hipGetLastError(), "right-factor pass launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
@@ -552,8 +564,10 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
     if (c.total_units) {
-        hipLaunchKernelGGL((right_factor_batch<KB, FP32>), dim3(c.total_units, passes), dim3(64),
-                           0, stream, c.members, c.first_unit, count, x, y, zp);
+        hipLaunchKernelGGL((right_factor_batch<KB, FP32>),
+                           dim3((c.total_units + kUnitWaves - 1) / kUnitWaves, passes),
+                           dim3(64 * kUnitWaves), 0, stream, c.members, c.first_unit, count,
+                           c.total_units, x, y, zp);
         hip_check(hipGetLastError(), "batched right-factor pass launch");
     }
     hipLaunchKernelGGL((row_tile_batch<KB, FP32>), dim3(c.total_tiles, passes),
@@ -592,6 +606,10 @@ template <class T> T* upload(const std::vector<T>& host, std::uint64_t& bytes) {
     hip_check(hipMemcpy(p, host.data(), n, hipMemcpyHostToDevice), "hipMemcpy host-to-device");
     return p;
 }
+
+// Relative cost of a row-tile entry, used to balance the waves of a tile:
+// a fixed latency share plus bytes loaded per lane plus decode work.
+constexpr std::uint64_t kEntryCost = 40;
 
 // Tiles of at most kTile indices between consecutive cuts.
 std::vector<std::pair<std::uint64_t, std::uint64_t>> make_tiles(std::vector<std::uint64_t> cuts) {
@@ -700,11 +718,13 @@ Plan build_plan(const qh_store_desc& d) {
             dvals.resize(at + m * nc);
             for (std::uint64_t i = 0; i < m; ++i)  // row-major in, column-major out
                 for (std::uint64_t c = 0; c < nc; ++c) dvals[at + c * m + i] = v[i * nc + c];
-            for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t) {
-                per_tile[t].push_back(
-                    {kDenseBit | di, std::uint32_t(row_tiles[t].first - r0), 0u, 0u, 0u});
-                per_cost[t].push_back(3 * nc);
-            }
+            for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t)
+                for (std::uint64_t c = 0; c < nc; c += kDenseCols) {
+                    const std::uint64_t cnt = std::min<std::uint64_t>(kDenseCols, nc - c);
+                    per_tile[t].push_back({kDenseBit | di, std::uint32_t(row_tiles[t].first - r0),
+                                           std::uint32_t(c), std::uint32_t(cnt), 0u});
+                    per_cost[t].push_back(kEntryCost + 8 * cnt);  // 8 bytes per lane per column
+                }
             dn_flops += 2 * m * nc;
             continue;
         }
@@ -747,7 +767,7 @@ Plan build_plan(const qh_store_desc& d) {
             s->left_chunks.back().push_back({std::uint32_t(a), std::uint32_t(rows), at});
             require(chunk_words + 8 < (std::uint64_t{1} << 32), "block chunk too large");
             per_tile[t].push_back({li, std::uint32_t(a), std::uint32_t(chunk_words), 0u, at});
-            per_cost[t].push_back(8 * r + chunk_words / 16);
+            per_cost[t].push_back(kEntryCost + 4 * r + chunk_words / 16);  // bytes per lane + decode
         }
         // right factor: one chunk (and one work unit) per column tile
         s->right_chunks.emplace_back();
@@ -949,7 +969,10 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
                         const EntryDev& ent = at(pl.entries, e);
                         if (ent.what & kDenseBit) {
                             const DenseDev& D = at(pl.denses, ent.what & ~kDenseBit);
-                            for (std::uint32_t c = 0; c < D.n; ++c)
+                            if (ent.cols == 0 || ent.cols > std::uint32_t(kDenseCols) ||
+                                ent.words + ent.cols > D.n)
+                                throw std::out_of_range("dense entry columns");
+                            for (std::uint32_t c = ent.words; c < ent.words + ent.cols; ++c)
                                 acc += at(pl.dense, D.off + std::uint64_t(c) * D.m + ent.row + i) *
                                        x[std::int64_t(at(pl.perm, D.col0 + c)) * k + p];
                             continue;

@@ -23,6 +23,7 @@ constexpr int kTile = 64;          // rows per row tile, columns per column tile
 constexpr int kRowWaves = 16;      // waves per row-tile workgroup
 constexpr int kStageWords = 1024;  // LDS words a wave stages at a time
 constexpr int kPassComponents = 16;  // components per right-factor pass (4 lanes each)
+constexpr int kUnitWaves = 4;        // right-factor units (one per wave) per workgroup
 
 // Low-rank block with rank > 0.  Right-factor partial sums of column tile s,
 // component j live in workspace row zbase + s * rank + j.
@@ -41,10 +42,12 @@ struct UnitDev {
 // Row-tile entry: kDenseBit | dense index, or low-rank block index.
 struct EntryDev {
     std::uint32_t what;
-    std::uint32_t row;         // dense: first tile row relative to the block
-    std::uint32_t words, pad;  // low-rank: words of the chunk (all components)
-    std::uint64_t chunk;       // low-rank: first word of the block's chunk for this tile
+    std::uint32_t row;    // dense: first tile row relative to the block
+    std::uint32_t words;  // low-rank: words of the chunk (all components); dense: first column
+    std::uint32_t cols;   // dense: columns of this entry (<= kDenseCols)
+    std::uint64_t chunk;  // low-rank: first word of the block's chunk for this tile
 };
+constexpr int kDenseCols = 16;  // columns of a diagonal block per row-tile entry
 constexpr std::uint32_t kDenseBit = 0x80000000u;
 
 // Row tile: rows [row0, row0+rows); entries [split[w], split[w+1]) are summed
