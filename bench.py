@@ -125,7 +125,13 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device-to-device once (a kernel of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--rhs", type=int, nargs="+", default=[1, 32])
     args = ap.parse_args()
+    if 1 not in args.rhs:
+        args.rhs = [1] + args.rhs  # the headline value is quoted at k=1
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -142,6 +148,13 @@ def main() -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
     log(f"building square_exp n={args.n}")
     w = W.square_exp(args.n)
     h = w.build()
@@ -152,7 +165,7 @@ def main() -> int:
         f"{4 * stats['code_words'] / 1e6:.1f} MB code stream, {args.copies} copies resident")
     lib = N.lib()
     results = {}
-    for k in (1, 32):
+    for k in args.rhs:
         x_np = w.rhs_matrix(k)
         x = torch.from_numpy(x_np).cuda()
         y = torch.empty_like(x)
@@ -228,20 +241,21 @@ def main() -> int:
                      "traffic": None,
                      "kernel": ["right_factor_pass", "row_tile_pass"][dominant],
                      "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
-        "fp32split_k1": head,
-        "fp64_k1": summary(1, "fp64"),
-        "fp32split_k32": summary(32, "fp32split"),
-        "fp64_k32": summary(32, "fp64"),
     }
+    for k in args.rhs:
+        for prec in ("fp32split", "fp64"):
+            line[f"{prec}_k{k}"] = summary(k, prec)
     if base is not None:
         line["cpu_baseline"] = {
             "value": round(b1["total"] / base["seconds"] / 1e9, 4), "unit": "GB/s",
             "cores": 1, "kind": base["kind"],
             "sample": f"{base['runs']} applies of the same store at k=1 (median "
                       f"{base['seconds'] * 1e3:.1f} ms/apply)"}
-    for key in ("fp32split_k1", "fp64_k1", "fp32split_k32", "fp64_k32"):
-        line[key] = {a: (float(f"{v:.6g}") if isinstance(v, float) else v)
-                     for a, v in line[key].items()}
+    for k in args.rhs:
+        for prec in ("fp32split", "fp64"):
+            key = f"{prec}_k{k}"
+            line[key] = {a: (float(f"{v:.6g}") if isinstance(v, float) else v)
+                         for a, v in line[key].items()}
     print(json.dumps(line), flush=True)
     return 0
 
