@@ -105,23 +105,26 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Copies `quads` 16-byte groups from global (16-byte aligned) to LDS.  Up to
 // four loads per lane are in flight before the first LDS write waits.
+// (A native clang vector type: an array of HIP's union-based uint4 is kept
+// in private scratch memory by the compiler.)
+typedef std::uint32_t quad_t __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void stage_quads(std::uint32_t* __restrict__ lds,
                                             const std::uint32_t* __restrict__ src,
                                             std::uint32_t quads, int lane) {
-    const uint4* s = reinterpret_cast<const uint4*>(src);
-    uint4* d = reinterpret_cast<uint4*>(lds);
+    const quad_t* s = reinterpret_cast<const quad_t*>(src);
+    quad_t* d = reinterpret_cast<quad_t*>(lds);
     for (std::uint32_t q0 = 0; q0 < quads; q0 += 256) {
-        uint4 v[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const std::uint32_t q = q0 + lane + 64 * i;
-            if (q < quads) v[i] = s[q];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const std::uint32_t q = q0 + lane + 64 * i;
-            if (q < quads) d[q] = v[i];
-        }
+        const std::uint32_t q = q0 + lane;
+        const quad_t zero = {0u, 0u, 0u, 0u};
+        const quad_t v0 = q < quads ? s[q] : zero;
+        const quad_t v1 = q + 64 < quads ? s[q + 64] : zero;
+        const quad_t v2 = q + 128 < quads ? s[q + 128] : zero;
+        const quad_t v3 = q + 192 < quads ? s[q + 192] : zero;
+        if (q < quads) d[q] = v0;
+        if (q + 64 < quads) d[q + 64] = v1;
+        if (q + 128 < quads) d[q + 128] = v2;
+        if (q + 192 < quads) d[q + 192] = v3;
     }
 }
 
@@ -319,11 +322,13 @@ __device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t
     const std::int64_t ldx = M.ldx, ldy = M.ldy;
     const int k = M.k;
 
-    const TileDev tile = M.tiles[tile_index];
+    // Fields read through the pointer: a local copy indexed by the wave
+    // number would be placed in private (scratch) memory.
+    const TileDev* __restrict__ tile = M.tiles + tile_index;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int kk = min(KB, k - p0);
-    const std::uint32_t R = tile.rows;
+    const std::uint32_t R = tile->rows;
     const bool active = lane < int(R);
     std::uint32_t* stage = stage_all[wave];
 
@@ -331,8 +336,8 @@ __device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t
 #pragma unroll
     for (int p = 0; p < KB; ++p) acc[p] = 0.0;
 
-    const std::uint32_t e_end = tile.split[wave + 1];
-    for (std::uint32_t e = tile.split[wave]; e < e_end; ++e) {
+    const std::uint32_t e_begin = tile->split[wave], e_end = tile->split[wave + 1];
+    for (std::uint32_t e = e_begin; e < e_end; ++e) {
         const EntryDev ent = entries[e];
         if (ent.what & kDenseBit) {
             // Up to kDenseCols columns of a diagonal block (column-major, one
@@ -448,7 +453,7 @@ __device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t
     }
 
     // Add the waves' partial rows in wave order, two right-hand sides at a time.
-    const std::int64_t yrow = active ? std::int64_t(perm[tile.row0 + lane]) * ldy + p0 : 0;
+    const std::int64_t yrow = active ? std::int64_t(perm[tile->row0 + lane]) * ldy + p0 : 0;
 #pragma unroll
     for (int pc = 0; pc < KB; pc += 2) {
         __syncthreads();
