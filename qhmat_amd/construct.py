@@ -5,9 +5,11 @@ The reference builds a store block by block with NumPy's SVD on the host
 are evaluated, factorised (batched FP64 SVD), rank-selected and checked on
 the device in one go; the width allocation and the packing stay in the
 library's C++ (the same code the host path uses).  Rank selection, budgets
-and feasibility checks follow matrix.py:213-245 exactly; only the SVD
-backend differs, so factors agree with the host construction to rounding,
-not bit for bit (the reference pins no SVD output either).
+and feasibility checks follow matrix.py:213-245; the SVD backend differs,
+so factors agree with the host construction to rounding, not bit for bit
+(the reference pins no SVD output either), and the truncation tail near the
+chosen rank is measured from the factors rather than summed from trailing
+singular values, which a batched device SVD resolves less accurately.
 """
 
 from __future__ import annotations
@@ -104,10 +106,27 @@ def build_on_device(points: np.ndarray, kernel_name: str, *, tolerance: float,
             has = ok.any(dim=1)
             first = torch.where(has, ok.float().argmax(dim=1) + 1, torch.zeros_like(has, dtype=torch.long))
             rank = torch.minimum(first, (sigma > 0).sum(dim=1))
+            cols = torch.arange(kmin, device=a.device)[None, :]
+
+            def truncated(r):
+                keep = (cols < r[:, None]).to(sigma.dtype)
+                return a - (u * (sigma * keep)[:, None, :]) @ vh
+
+            # A batched device SVD resolves the trailing singular values less
+            # accurately than LAPACK, which can move the first rank meeting the
+            # allowance by one.  tail[r] is ||A - A_r||_F, so near the cut it
+            # is evaluated directly from the factors (accurate to rounding of
+            # ||A||) and the first rank in the window that meets the
+            # allowance is taken.
+            window = range(-2, 3)
+            chosen = rank.clone()
+            for d in reversed(window):
+                r = torch.clamp(rank + d, 1, cap)
+                fits = torch.linalg.matrix_norm(truncated(r)) <= allowance
+                chosen = torch.where(has & fits & (r <= (sigma > 0).sum(dim=1)), r, chosen)
+            rank = chosen
             # residual max-norm at the chosen rank
-            keep = (torch.arange(kmin, device=a.device)[None, :] < rank[:, None]).to(sigma.dtype)
-            approx = (u * (sigma * keep)[:, None, :]) @ vh
-            residual = (a - approx).abs().amax(dim=(1, 2))
+            residual = truncated(rank).abs().amax(dim=(1, 2))
             budget_h, has_h = budget.cpu().numpy(), has.cpu().numpy()
             rank_h, res_h = rank.cpu().numpy(), residual.cpu().numpy()
             rmax = int(rank_h.max()) if rank_h.size else 0
@@ -134,7 +153,7 @@ def build_on_device(points: np.ndarray, kernel_name: str, *, tolerance: float,
                                           "is infeasible even at the highest bit width")
                 results[i] = ("lowrank", (u_h[t, :, :rk], sig, v_h[t, :, :rk], plan.bits,
                                           float(res_h[t])))
-            del a, u, sigma, vh, approx
+            del a, u, sigma, vh
 
     native = NativeHMatrix(n, part.permutation)
     for i, rect in enumerate(part.blocks):
