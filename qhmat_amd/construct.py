@@ -96,6 +96,11 @@ def build_on_device(points: np.ndarray, kernel_name: str, *, tolerance: float,
             a = gather(chunk, m, c)  # (B, m, c)
             budget = tolerance * a.abs().amax(dim=(1, 2))
             u, sigma, vh = torch.linalg.svd(a, full_matrices=False)
+            # A batched device SVD need not return the singular values in
+            # order; rank selection and the allocation expect them descending.
+            sigma, order = torch.sort(sigma, dim=1, descending=True)
+            u = torch.gather(u, 2, order[:, None, :].expand(-1, u.shape[1], -1))
+            vh = torch.gather(vh, 1, order[:, :, None].expand(-1, -1, vh.shape[2]))
             kmin = sigma.shape[1]
             # tail[:, r] = Frobenius norm of the part discarded at rank r
             sq = torch.flip(torch.cumsum(torch.flip(sigma * sigma, [1]), 1), [1])
