@@ -40,15 +40,22 @@ def log(msg: str) -> None:
 
 
 def algorithmic_bytes(stats: dict, k: int) -> dict:
-    """Bytes an apply must move at minimum, by pass (device formats)."""
+    """Bytes an apply must move at minimum: the packed words as the reference
+    stores them (per block and component, not per tile), two FP64 scales and
+    a width per component, the diagonal blocks, x and y once and the
+    permutation (8 bytes per index, as the reference holds it).  Split by
+    pass: pass 1 reads the right factors, the fused blocks' left factors, the
+    diagonal blocks and x; pass 2 the remaining left factors and writes y."""
     n = stats["order"]
-    meta = 20 * stats["ncomponents"]  # two FP64 scales + an int32 width
-    right = 4 * stats["right_code_words"] + 4 * stats["ncomponents"] + 8 * n * k + 4 * n
-    left = (4 * stats["left_code_words"] + 16 * stats["ncomponents"] + 8 * stats["dense_values"]
-            + 8 * n * k + 4 * n)
-    total = 4 * stats["code_words"] + meta + 8 * stats["dense_values"] + 16 * n * k + 4 * n
-    return {"total": total, "code_stream": 4 * stats["code_words"], "pass0": right,
-            "pass1": left}
+    comps = stats["ncomponents"]
+    fused = stats["fused_components"]
+    w1, w2 = stats["pass_code_words"]
+    total = 4 * stats["code_words"] + 20 * comps + 8 * stats["dense_values"] + 16 * n * k + 8 * n
+    pass1 = 4 * w1 + 20 * fused + 4 * (comps - fused) + 8 * stats["dense_values"] + 8 * n * k
+    pass2 = 4 * w2 + 16 * (comps - fused) + 8 * n * k + 8 * n
+    assert pass1 + pass2 == total
+    return {"total": total, "code_stream": 4 * stats["code_words"], "pass0": pass1,
+            "pass1": pass2}
 
 
 def flops(stats: dict, k: int) -> dict:
@@ -239,7 +246,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": ["right_factor_pass", "row_tile_pass"][dominant],
+                     "kernel": ["apply_pass1", "apply_pass2"][dominant],
                      "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
     }
     for k in args.rhs:

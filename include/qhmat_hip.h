@@ -89,7 +89,11 @@ typedef struct {
     uint64_t device_bytes;        /* everything resident on the device        */
     uint64_t lowrank_flops_per_rhs; /* sum 2 r (m+n)                          */
     uint64_t dense_flops_per_rhs;   /* sum 2 m n over dense blocks            */
-    uint64_t tiles, items, entries;
+    uint64_t tiles;               /* row tiles (at most 64 rows each)         */
+    uint64_t items;               /* work items of both passes                */
+    uint64_t right_waves;         /* pass-1 waves streaming right factors     */
+    uint64_t pass_code_words[2];  /* of code_words: read by pass 1 / pass 2    */
+    uint64_t fused_components;    /* components of blocks applied in pass 1   */
     int32_t device;
 } qh_store_stats;
 

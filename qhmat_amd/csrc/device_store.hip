@@ -1,30 +1,33 @@
 // MI355X (gfx950) apply of a qhmat H-matrix, y = H x.
 //
-// The packed store is uploaded once in the tile-major layout described in
-// device_store.hpp.  An apply is two launches on the caller's stream, both
+// The packed store is uploaded once as a list of work items (layout in
+// device_store.hpp).  An apply is two launches on the caller's stream, both
 // deterministic (fixed summation order, one writer per output element, no
 // floating-point atomics):
 //
-//  1. right-factor pass: one wave per (low-rank block, column tile).  The
-//     wave streams the tile's chunk into LDS with 16-byte loads and the
-//     tile's x rows (gathered through the permutation) next to it; four
-//     lanes share each component, decode their codes from LDS
-//     (v_alignbit_b32 + v_bfe_i32), multiply by x and are reduced in the
-//     quad.  Output: partial dot products
-//         zp[zbase + s*rank + j][p] = sum_{c in tile s} codeR(c, j) x[c, p].
-//  2. row-tile pass: one workgroup of eight waves per row tile.  The waves
-//     split the tile's blocks in insertion order; a wave streams each
-//     block's chunk for the tile into LDS and every lane (one row) decodes
-//     its code of each component and accumulates
-//         y[row, p] += sum_j codeL(row, j) * (sL_j sR_j sum_s zp[.][p]),
-//     plus D[row, :] x for the diagonal block.  The eight partial rows are
-//     added in wave order and written once to y[perm[row], p].
+//  pass 1  workgroups [0, rgroups) are independent waves streaming kRight
+//          items: partial sums z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c]
+//          into the workspace.  Workgroups from rgroups on own one row tile
+//          each: their waves sum the tile's diagonal (kDense) and fused
+//          (kFused: z and U z inside the item) contributions, and the
+//          waves' rows are added in wave order into the workspace.
+//  pass 2  one workgroup per row tile: the kLeft items (U z with z summed
+//          over the block's column tiles), added to the pass-1 rows in wave
+//          order and written once to y[perm[row]].
 //
-// FP64 mode computes in FP64 throughout.  FP32 split mode converts codes of
-// at most 24 bits and x to FP32 (exact for the codes), splits wider codes
-// into a signed high half and an unsigned 16-bit low half accumulated apart,
-// sums each block's contribution to a row in FP32 and everything across
-// blocks, the scales and the diagonal in FP64.
+// Every wave walks a contiguous list of items with the next item's data
+// (<= 4 KiB, four 16-byte loads per lane), component widths, scales and
+// inputs in flight while the current item is decoded from LDS: the record
+// list is read once (lanes = items), the permutation rows of the item after
+// next are requested with the next item's data, so no load waits on another
+// inside the loop.
+//
+// Decoding: a code is v_alignbit_b32 of two staged words + v_bfe_i32.  FP64
+// mode computes in FP64 throughout.  FP32 split mode converts codes of at
+// most 24 bits and x to FP32 (exact for the codes), splits wider codes into
+// a signed high half and an unsigned 16-bit low half accumulated apart, sums
+// each item's contribution to a row in FP32 and everything across items, the
+// scales and the diagonal in FP64.
 //
 // Reference: src/qhmat/cuda.cu:64-164 (one thread per output entry scanning
 // every block, store uploaded per call) and core.cpp:360-390 (FP64 loop nest).
@@ -103,161 +106,465 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Copies `quads` 16-byte groups from global (16-byte aligned) to LDS.  Up to
-// four loads per lane are in flight before the first LDS write waits.
-// (A native clang vector type: an array of HIP's union-based uint4 is kept
-// in private scratch memory by the compiler.)
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ std::uint32_t wave_scan(std::uint32_t v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const std::uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// 16-byte group as a native clang vector (an array of HIP's union-based
+// uint4 would be kept in private scratch memory by the compiler).
 typedef std::uint32_t quad_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void stage_quads(std::uint32_t* __restrict__ lds,
-                                            const std::uint32_t* __restrict__ src,
-                                            std::uint32_t quads, int lane) {
-    const quad_t* s = reinterpret_cast<const quad_t*>(src);
-    quad_t* d = reinterpret_cast<quad_t*>(lds);
-    for (std::uint32_t q0 = 0; q0 < quads; q0 += 256) {
-        const std::uint32_t q = q0 + lane;
-        const quad_t zero = {0u, 0u, 0u, 0u};
-        const quad_t v0 = q < quads ? s[q] : zero;
-        const quad_t v1 = q + 64 < quads ? s[q + 64] : zero;
-        const quad_t v2 = q + 128 < quads ? s[q + 128] : zero;
-        const quad_t v3 = q + 192 < quads ? s[q + 192] : zero;
-        if (q < quads) d[q] = v0;
-        if (q + 64 < quads) d[q + 64] = v1;
-        if (q + 128 < quads) d[q + 128] = v2;
-        if (q + 192 < quads) d[q + 192] = v3;
+template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
+
+// LDS of one wave: the current item's data (plus one readable quad past the
+// end) and its inputs (x rows, or the workspace partial sums of kLeft).
+template <int KB> constexpr int wave_lds_bytes() { return (kItemQuads + 1) * 16 + kTile * KB * 8; }
+
+// Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
+// items of the group).
+struct Item {
+    std::uint32_t quad, quads, kind, count, comp0, col, cols, zrow, zstride, nseg;
+};
+
+__device__ __forceinline__ Item item_at(const quad_t& r0, const quad_t& r1, std::uint32_t slot) {
+    Item f;
+    f.quad = __builtin_amdgcn_readlane(r0.x, slot);
+    const std::uint32_t packed = __builtin_amdgcn_readlane(r0.y, slot);
+    f.quads = packed & 0xFFFFu;
+    f.kind = (packed >> 16) & 0xFFu;
+    f.count = min(packed >> 24, std::uint32_t(kItemComponents));
+    f.comp0 = __builtin_amdgcn_readlane(r0.z, slot);
+    f.col = __builtin_amdgcn_readlane(r0.w, slot);
+    f.cols = min(std::uint32_t(__builtin_amdgcn_readlane(r1.x, slot)), std::uint32_t(kTile));
+    f.zrow = __builtin_amdgcn_readlane(r1.y, slot);
+    f.zstride = __builtin_amdgcn_readlane(r1.z, slot);
+    f.nseg = max(std::uint32_t(__builtin_amdgcn_readlane(r1.w, slot)), 1u);
+    f.quads = min(f.quads, std::uint32_t(kItemQuads));
+    return f;
+}
+
+// Columns of x an item reads (lanes = columns).
+__device__ __forceinline__ std::uint32_t x_columns(const Item& f) {
+    return f.kind == kDense ? min(f.count, std::uint32_t(kDenseColumns))
+                            : (f.kind == kLeft ? 0u : f.cols);
+}
+
+// Workspace partial sums a kLeft item reads (lanes = (component, column tile)).
+__device__ __forceinline__ std::uint32_t z_pairs(const Item& f) {
+    return f.kind == kLeft ? min(f.count * f.nseg, 64u) : 0u;
+}
+
+// Registers of the item being fetched.
+template <int KB> struct Fetch {
+    quad_t q[kItemQuads / 64];
+    std::uint32_t bits;  // lanes = components
+    double scale;        // lanes = components (kFused, kLeft)
+    double in[KB];       // x rows (lanes = columns) or partial sums (lanes = pairs)
+};
+
+__device__ __forceinline__ std::int32_t perm_row(const MemberView& M, const Item& f, int lane) {
+    return lane < int(x_columns(f)) ? M.perm[f.col + lane] : 0;
+}
+
+template <int KB>
+__device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::int32_t prow,
+                                      int lane, int p0, int kk, Fetch<KB>& F) {
+    const quad_t* src = reinterpret_cast<const quad_t*>(M.words) + f.quad;
+    const quad_t zero = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int t = 0; t < kItemQuads / 64; ++t)
+        F.q[t] = std::uint32_t(lane + 64 * t) < f.quads ? src[lane + 64 * t] : zero;
+    const bool comp = f.kind != kDense && lane < int(f.count);
+    F.bits = comp ? std::uint32_t(M.widths[f.comp0 + lane]) : 0u;
+    F.scale = (comp && f.kind != kRight) ? M.cscale[f.comp0 + lane] : 0.0;
+    const std::int64_t k = M.k;
+    if (f.kind == kLeft) {
+        const std::uint32_t s = lane % f.nseg, j = lane / f.nseg;
+        const bool mine = std::uint32_t(lane) < z_pairs(f);
+        const double* zr = M.zp + (std::uint64_t(f.zrow) + std::uint64_t(s) * f.zstride + j) * k + p0;
+#pragma unroll
+        for (int p = 0; p < KB; ++p) F.in[p] = (mine && p < kk) ? zr[p] : 0.0;
+    } else {
+        const bool mine = lane < int(x_columns(f));
+        const double* xr = M.x + std::int64_t(prow) * M.ldx + p0;
+#pragma unroll
+        for (int p = 0; p < KB; ++p) F.in[p] = (mine && p < kk) ? xr[p] : 0.0;
     }
 }
 
-// ---------------------------------------------------------------------------
-// Pass 1: right factor, one wave per (block, column tile).
-template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
-
+// Sum over one column tile of a right-factor stream: lanes (g, q) hold
+// component g of the round, columns q, q+4, ...; the quad's four partial
+// sums are added (lanes 4g..4g+3 all end with the total).
 template <int KB, bool FP32>
-__device__ __forceinline__ void right_factor_unit(const MemberView& M, std::uint32_t unit, int p0,
-                                                  std::uint32_t* __restrict__ stage,
-                                                  real_t<FP32> (*xs)[KB]) {
+__device__ __forceinline__ void right_sums(const std::uint32_t* __restrict__ stage,
+                                           const real_t<FP32>* __restrict__ xr,
+                                           std::uint32_t cols, std::uint32_t bits,
+                                           std::uint32_t off, bool mine, int q, double (&tot)[KB]) {
     using Real = real_t<FP32>;
-    const std::int32_t* __restrict__ widths = M.widths;
-    const std::uint32_t* __restrict__ words = M.words;
-    const std::int32_t* __restrict__ perm = M.perm;
-    const double* __restrict__ x = M.x;
-    double* __restrict__ zp = M.zp;
-    const std::int64_t ldx = M.ldx;
-    const int k = M.k;
-
-    const UnitDev u = M.units[unit];
-    const BlockDev B = M.blocks[u.block];
-    const int lane = threadIdx.x & 63;
-    const int kk = min(KB, k - p0);
-    const std::uint32_t C = u.cols;
-
-    // Request everything before waiting on anything: the first pass's widths,
-    // the permutation rows, the whole chunk when it fits the stage buffer,
-    // then the x rows.
-    const bool whole = u.words + 5 <= std::uint32_t(kStageWords);
-    const std::uint32_t delta0 = std::uint32_t(u.chunk & 3u);
-    const std::uint32_t jn0 = min(std::uint32_t(kPassComponents), B.rank);
-    std::uint32_t wl = lane < int(jn0) ? std::uint32_t(widths[B.comp_base + lane]) : 0u;
-    const std::int64_t xrow = lane < int(C) ? std::int64_t(perm[u.col0 + lane]) * ldx + p0 : 0;
-    if (whole) stage_quads(stage, words + (u.chunk - delta0), (u.words + delta0 + 1 + 3) >> 2, lane);
-    if (lane < int(C)) {
+    const std::uint32_t* sw = stage + off;
+    if (!FP32 || bits <= 24) {
+        Real part[KB];
 #pragma unroll
-        for (int p = 0; p < KB; ++p) xs[lane][p] = p < kk ? Real(x[xrow + p]) : Real(0);
-    }
-
-    const int g = lane >> 2;  // component slot of this lane in a pass
-    const int q = lane & 3;   // columns q, q+4, q+8, ... of the tile
-    std::uint32_t done = 0;   // chunk words of the components already processed
-    for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
-        const std::uint32_t jn = min(std::uint32_t(kPassComponents), B.rank - j0);
-        // widths and stream offsets of this pass's components (prefix scan)
-        if (j0 > 0) wl = lane < int(jn) ? std::uint32_t(widths[B.comp_base + j0 + lane]) : 0u;
-        const std::uint32_t nwl = stream_words(C, wl);
-        std::uint32_t incl = nwl;
+        for (int p = 0; p < KB; ++p) part[p] = Real(0);
+        if (mine) {
+            for (std::uint32_t c = q; c < cols; c += 4) {
+                const Real code = Real(decode_code(sw, c, bits));
 #pragma unroll
-        for (int off = 1; off < kPassComponents; off <<= 1) {
-            const std::uint32_t t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        const std::uint32_t total = __shfl(incl, kPassComponents - 1, 64);
-        const std::uint32_t bits = __shfl(wl, g & (kPassComponents - 1), 64);
-        const std::uint32_t seg = __shfl(incl - nwl, g & (kPassComponents - 1), 64);
-
-        std::uint32_t base = delta0 + done;  // first word of this pass in the stage buffer
-        if (!whole) {
-            const std::uint64_t first = u.chunk + done;
-            base = std::uint32_t(first & 3u);
-            stage_quads(stage, words + (first - base), (total + base + 1 + 3) >> 2, lane);
-        }
-        wave_lds_sync();
-
-        double tot[KB];
-        const bool mine = g < int(jn);
-        const std::uint32_t* sw = stage + base + seg;
-        if (!FP32 || bits <= 24) {
-            Real part[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) part[p] = Real(0);
-            if (mine) {
-                for (std::uint32_t c = q; c < C; c += 4) {
-                    const Real code = Real(decode_code(sw, c, bits));
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) part[p] = fma(code, xs[c][p], part[p]);
-                }
+                for (int p = 0; p < KB; ++p) part[p] = fma(code, xr[c * KB + p], part[p]);
             }
+        }
 #pragma unroll
-            for (int p = 0; p < KB; ++p) {
-                Real v = part[p];
-                v += __shfl_xor(v, 1, 64);
-                v += __shfl_xor(v, 2, 64);
-                tot[p] = double(v);
-            }
-        } else {
-            // 25..32-bit codes: signed high half and unsigned low 16 bits,
-            // both exact in FP32, summed apart and joined in FP64.
-            float hi[KB], lo[KB];
+        for (int p = 0; p < KB; ++p) {
+            Real v = part[p];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            tot[p] = double(v);
+        }
+    } else {
+        // 25..32-bit codes: signed high half and unsigned low 16 bits, both
+        // exact in FP32, summed apart and joined in FP64.
+        float hi[KB], lo[KB];
 #pragma unroll
-            for (int p = 0; p < KB; ++p) hi[p] = lo[p] = 0.0f;
-            for (std::uint32_t c = q; c < C; c += 4) {
+        for (int p = 0; p < KB; ++p) hi[p] = lo[p] = 0.0f;
+        if (mine) {
+            for (std::uint32_t c = q; c < cols; c += 4) {
                 const int code = decode_code(sw, c, bits);
                 const float h = float(code >> 16), l = float(code & 0xFFFF);
 #pragma unroll
                 for (int p = 0; p < KB; ++p) {
-                    hi[p] = fmaf(h, float(xs[c][p]), hi[p]);
-                    lo[p] = fmaf(l, float(xs[c][p]), lo[p]);
+                    hi[p] = fmaf(h, float(xr[c * KB + p]), hi[p]);
+                    lo[p] = fmaf(l, float(xr[c * KB + p]), lo[p]);
                 }
             }
-#pragma unroll
-            for (int p = 0; p < KB; ++p) {
-                float h = hi[p], l = lo[p];
-                h += __shfl_xor(h, 1, 64);
-                h += __shfl_xor(h, 2, 64);
-                l += __shfl_xor(l, 1, 64);
-                l += __shfl_xor(l, 2, 64);
-                tot[p] = double(h) * 65536.0 + double(l);
-            }
         }
-        if (mine && q == 0) {
-            double* out = zp + (std::uint64_t(u.zrow) + j0 + g) * std::uint64_t(k) + p0;
 #pragma unroll
-            for (int p = 0; p < KB; ++p)
-                if (p < kk) out[p] = tot[p];
+        for (int p = 0; p < KB; ++p) {
+            float h = hi[p], l = lo[p];
+            h += __shfl_xor(h, 1, 64);
+            h += __shfl_xor(h, 2, 64);
+            l += __shfl_xor(l, 1, 64);
+            l += __shfl_xor(l, 2, 64);
+            tot[p] = double(h) * 65536.0 + double(l);
         }
-        done += total;
-        if (!whole) wave_lds_sync();  // the next pass restages
     }
 }
 
-// kUnitWaves units per workgroup, one per wave (a one-wave workgroup would
-// leave the per-CU workgroup limit, not the waves, bounding occupancy).
+// Row contribution of one left-factor component: lanes are rows; zs is the
+// component's scaled input (lane-uniform).
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kUnitWaves) void right_factor_pass(MemberView M) {
-    __shared__ __align__(16) std::uint32_t stage[kUnitWaves][kStageWords + 8];
-    __shared__ real_t<FP32> xs[kUnitWaves][kTile][KB];
+__device__ __forceinline__ void left_component(const std::uint32_t* __restrict__ stage,
+                                               std::uint32_t off, std::uint32_t bits, bool active,
+                                               int lane, const real_t<FP32> (&zs)[KB],
+                                               double (&acc)[KB], float (&blk)[KB]) {
+    const int code = active ? decode_code(stage + off, lane, bits) : 0;
+    if (!FP32) {
+        const double cd = double(code);
+#pragma unroll
+        for (int p = 0; p < KB; ++p) acc[p] = fma(cd, double(zs[p]), acc[p]);
+    } else if (bits <= 24) {
+        const float cf = float(code);
+#pragma unroll
+        for (int p = 0; p < KB; ++p) blk[p] = fmaf(cf, float(zs[p]), blk[p]);
+    } else {
+        const float h = float(code >> 16), l = float(code & 0xFFFF);
+#pragma unroll
+        for (int p = 0; p < KB; ++p) {
+            blk[p] = fmaf(h, float(zs[p]) * 65536.0f, blk[p]);
+            blk[p] = fmaf(l, float(zs[p]), blk[p]);
+        }
+    }
+}
+
+// One wave's items [b, e).  Row contributions (kDense, kFused, kLeft) are
+// added to acc (lanes = the R rows of the tile); kRight items write their
+// partial sums to the workspace.
+template <int KB, bool FP32>
+__device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b, std::uint32_t e,
+                                          int p0, int kk, std::uint32_t R,
+                                          std::uint8_t* __restrict__ wave_lds, double (&acc)[KB]) {
+    using Real = real_t<FP32>;
+    const int lane = threadIdx.x & 63;
+    const bool active = lane < int(R);
+    std::uint32_t* stage = reinterpret_cast<std::uint32_t*>(wave_lds);
+    quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds);
+    double* xs = reinterpret_cast<double*>(wave_lds + (kItemQuads + 1) * 16);
+    Real* xr = reinterpret_cast<Real*>(xs);
+    const std::int64_t k = M.k;
+
+    for (std::uint32_t g0 = b; g0 < e; g0 += 64) {
+        const std::uint32_t n = min(64u, e - g0);
+        quad_t r0 = {0u, 0u, 0u, 0u}, r1 = {0u, 0u, 0u, 0u};
+        if (std::uint32_t(lane) < n) {
+            const quad_t* rp = reinterpret_cast<const quad_t*>(M.items + g0 + lane);
+            r0 = rp[0];
+            r1 = rp[1];
+        }
+        Fetch<KB> F;
+        fetch<KB>(M, item_at(r0, r1, 0), perm_row(M, item_at(r0, r1, 0), lane), lane, p0, kk, F);
+        std::int32_t prow = n > 1 ? perm_row(M, item_at(r0, r1, 1), lane) : 0;
+
+        for (std::uint32_t s = 0; s < n; ++s) {
+            const Item f = item_at(r0, r1, s);
+            // Hand the fetched registers to LDS and keep the per-component ones.
+#pragma unroll
+            for (int t = 0; t < kItemQuads / 64; ++t)
+                if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
+            if (f.kind == kDense || f.kind == kLeft) {
+                if (std::uint32_t(lane) < max(x_columns(f), z_pairs(f))) {
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+                }
+            } else if (lane < int(f.cols)) {
+#pragma unroll
+                for (int p = 0; p < KB; ++p) xr[lane * KB + p] = Real(F.in[p]);
+            }
+            const std::uint32_t bits_l = F.bits;
+            const double scale_l = F.scale;
+            // Request the next item, and the permutation rows of the one after.
+            if (s + 1 < n) {
+                fetch<KB>(M, item_at(r0, r1, s + 1), prow, lane, p0, kk, F);
+                if (s + 2 < n) prow = perm_row(M, item_at(r0, r1, s + 2), lane);
+            }
+            wave_lds_sync();
+
+            if (f.kind == kDense) {
+                // lanes = rows; column c of the item at stage[2 (c R + row)]
+                const double* dv = reinterpret_cast<const double*>(stage);
+                const std::uint32_t cnt = min(f.count, std::uint32_t(kDenseColumns));
+#pragma unroll
+                for (int c = 0; c < kDenseColumns; ++c) {
+                    if (std::uint32_t(c) < cnt) {
+                        const double v = dv[c * R + lane];
+#pragma unroll
+                        for (int p = 0; p < KB; ++p) acc[p] = fma(v, xs[c * KB + p], acc[p]);
+                    }
+                }
+            } else if (f.kind == kLeft) {
+                // z of each component: its column tiles' partial sums in order
+                double z[KB];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) z[p] = 0.0;
+                const std::uint32_t nseg = f.nseg;
+                if (std::uint32_t(lane) < f.count && (lane + 1) * nseg <= 64u) {
+                    for (std::uint32_t sg = 0; sg < nseg; ++sg) {
+#pragma unroll
+                        for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
+                    }
+                }
+                Real zs_l[KB];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
+                const std::uint32_t nw = std::uint32_t(lane) < f.count ? stream_words(R, bits_l) : 0u;
+                const std::uint32_t off_l = wave_scan(nw, lane) - nw;
+                float blk[KB];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
+                for (std::uint32_t j = 0; j < f.count; ++j) {
+                    Real zs[KB];
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) zs[p] = lane_value(zs_l[p], j);
+                    left_component<KB, FP32>(stage, __builtin_amdgcn_readlane(off_l, j),
+                                             __builtin_amdgcn_readlane(bits_l, j), active, lane,
+                                             zs, acc, blk);
+                }
+                if (FP32) {
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
+                }
+            } else {
+                // kFused / kRight: right-factor sums in rounds of 16 components
+                const std::uint32_t nwr =
+                    std::uint32_t(lane) < f.count ? stream_words(f.cols, bits_l) : 0u;
+                const std::uint32_t incl_r = wave_scan(nwr, lane);
+                const std::uint32_t roff_l = incl_r - nwr;
+                const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
+                const std::uint32_t nwl =
+                    std::uint32_t(lane) < f.count ? stream_words(R, bits_l) : 0u;
+                const std::uint32_t loff_l = rtotal + wave_scan(nwl, lane) - nwl;
+                const int g = lane >> 2, q = lane & 3;
+                float blk[KB];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
+                for (std::uint32_t j0 = 0; j0 < f.count; j0 += 16) {
+                    const std::uint32_t jn = min(16u, f.count - j0);
+                    const int src = int(j0) + g;
+                    const std::uint32_t bits_g = __shfl(bits_l, src, 64);
+                    const std::uint32_t off_g = __shfl(roff_l, src, 64);
+                    const bool mine = std::uint32_t(g) < jn;
+                    double tot[KB];
+                    right_sums<KB, FP32>(stage, xr, f.cols, bits_g, off_g, mine, q, tot);
+                    if (f.kind == kRight) {
+                        if (mine && q == 0) {
+                            double* out = M.zp + (std::uint64_t(f.zrow) + j0 + g) * std::uint64_t(k) + p0;
+#pragma unroll
+                            for (int p = 0; p < KB; ++p)
+                                if (p < kk) out[p] = tot[p];
+                        }
+                        continue;
+                    }
+                    const double scale_g = __shfl(scale_l, src, 64);
+                    Real zs_g[KB];
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) zs_g[p] = Real(scale_g * tot[p]);
+                    for (std::uint32_t j = 0; j < jn; ++j) {
+                        Real zs[KB];
+#pragma unroll
+                        for (int p = 0; p < KB; ++p) zs[p] = lane_value(zs_g[p], 4 * j);
+                        left_component<KB, FP32>(stage, __builtin_amdgcn_readlane(loff_l, j0 + j),
+                                                 __builtin_amdgcn_readlane(bits_l, j0 + j), active,
+                                                 lane, zs, acc, blk);
+                    }
+                }
+                if (FP32 && f.kind == kFused) {
+#pragma unroll
+                    for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
+                }
+            }
+            wave_lds_sync();  // the next item overwrites the stage
+        }
+    }
+}
+
+// The row sums of a tile are a balanced binary tree over its kWaves runs of
+// items, each run summed in item order from zero:
+//   ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) + ...
+// A W-wave workgroup gives each wave kWaves/W consecutive runs and forms the
+// lower levels of the tree in registers, the rest in wave 0; the result is
+// the same bits for every W (so for every right-hand-side count).
+template <int KB, bool FP32, int W>
+__device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32_t* split, int wave,
+                                          int p0, int kk, std::uint32_t R, std::uint8_t* wave_lds,
+                                          double (&total)[KB]) {
+    constexpr int step = kWaves / W;
+    double pair[KB];
+#pragma unroll
+    for (int i = 0; i < step; ++i) {
+        double acc[KB];
+#pragma unroll
+        for (int p = 0; p < KB; ++p) acc[p] = 0.0;
+        run_items<KB, FP32>(M, split[wave * step + i], split[wave * step + i + 1], p0, kk, R,
+                            wave_lds, acc);
+#pragma unroll
+        for (int p = 0; p < KB; ++p) {
+            if (i == 0) total[p] = acc[p];
+            else if (i == 1) total[p] += acc[p];
+            else if (i == 2) { pair[p] = total[p]; total[p] = acc[p]; }
+            else total[p] = pair[p] + (total[p] + acc[p]);
+        }
+    }
+}
+
+// Completes the tree over the W wave totals in wave 0 (two right-hand sides
+// per round through LDS), adds it to `base` and hands the result to `store`.
+template <int KB, int W, class Store>
+__device__ __forceinline__ void combine_rows(const double (&total)[KB], const double (&base)[KB],
+                                             std::uint8_t* lds, int wave, int lane, Store store) {
+    double (*partial)[2][64] = reinterpret_cast<double (*)[2][64]>(lds);
+#pragma unroll
+    for (int pc = 0; pc < KB; pc += 2) {
+        __syncthreads();
+        if (wave > 0) {
+            partial[wave - 1][0][lane] = total[pc];
+            partial[wave - 1][1][lane] = pc + 1 < KB ? total[pc + 1] : 0.0;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double v0[W], v1[W];
+            v0[0] = total[pc];
+            v1[0] = pc + 1 < KB ? total[pc + 1] : 0.0;
+#pragma unroll
+            for (int w = 1; w < W; ++w) {
+                v0[w] = partial[w - 1][0][lane];
+                v1[w] = partial[w - 1][1][lane];
+            }
+#pragma unroll
+            for (int width = 1; width < W; width *= 2) {
+#pragma unroll
+                for (int i = 0; i + width < W; i += 2 * width) {
+                    v0[i] += v0[i + width];
+                    v1[i] += v1[i + width];
+                }
+            }
+            store(pc, base[pc] + v0[0], pc + 1 < KB ? base[pc + 1] + v1[0] : 0.0);
+        }
+    }
+}
+
+template <int KB, bool FP32, int W>
+__device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t group, int p0,
+                                            std::uint8_t* lds) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const std::uint32_t unit = blockIdx.x * kUnitWaves + wave;
-    if (unit >= M.units_count) return;
-    right_factor_unit<KB, FP32>(M, unit, blockIdx.y * KB, stage[wave], xs[wave]);
+    const int lane = threadIdx.x & 63;
+    const int kk = min(KB, M.k - p0);
+    std::uint8_t* mine = lds + wave * wave_lds_bytes<KB>();
+    double total[KB];
+    if (group < M.rgroups) {  // independent right-factor waves
+        const std::uint32_t w = group * W + wave;
+        if (w < M.rwaves) {
+#pragma unroll
+            for (int p = 0; p < KB; ++p) total[p] = 0.0;
+            run_items<KB, FP32>(M, M.rsplit[w], M.rsplit[w + 1], p0, kk, 0, mine, total);
+        }
+        return;
+    }
+    const TileDev* __restrict__ tile = M.tiles + (group - M.rgroups);
+    const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
+    wave_rows<KB, FP32, W>(M, tile->split1, wave, p0, kk, R, mine, total);
+    double zero[KB];
+#pragma unroll
+    for (int p = 0; p < KB; ++p) zero[p] = 0.0;
+    double* rows = M.zp + (M.z_rows + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+    combine_rows<KB, W>(total, zero, lds, wave, lane, [&](int pc, double a0, double a1) {
+        if (lane < int(R)) {
+            if (pc < kk) rows[pc] = a0;
+            if (pc + 1 < kk) rows[pc + 1] = a1;
+        }
+    });
+}
+
+template <int KB, bool FP32, int W>
+__device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t, int p0,
+                                           std::uint8_t* lds) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int kk = min(KB, M.k - p0);
+    const TileDev* __restrict__ tile = M.tiles + t;
+    const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
+    const bool active = lane < int(R);
+    // pass-1 rows of this tile, requested first
+    double base[KB];
+    const double* rows = M.zp + (M.z_rows + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+#pragma unroll
+    for (int p = 0; p < KB; ++p) base[p] = (wave == 0 && active && p < kk) ? rows[p] : 0.0;
+    const std::int64_t yrow = active ? std::int64_t(M.perm[tile->row0 + lane]) * M.ldy + p0 : 0;
+    double total[KB];
+    wave_rows<KB, FP32, W>(M, tile->split2, wave, p0, kk, R, lds + wave * wave_lds_bytes<KB>(),
+                           total);
+    double* y = M.y;
+    combine_rows<KB, W>(total, base, lds, wave, lane, [&](int pc, double a0, double a1) {
+        if (active) {
+            if (pc < kk) y[yrow + pc] = a0;
+            if (pc + 1 < kk) y[yrow + pc + 1] = a1;
+        }
+    });
+}
+
+template <int KB, bool FP32, int W>
+__global__ __launch_bounds__(64 * W) void apply_pass1(MemberView M) {
+    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
+    pass1_group<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
+}
+
+template <int KB, bool FP32, int W>
+__global__ __launch_bounds__(64 * W) void apply_pass2(MemberView M) {
+    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
+    pass2_tile<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
 // The member owning global work index `index`: first[m] <= index < first[m+1].
@@ -273,7 +580,7 @@ __device__ __forceinline__ std::uint32_t locate_member(const std::uint32_t* __re
 }
 
 // Batched: the work of every member of one launch, x / y / workspace given
-// as offsets (xoff, yoff, zoff) from the call's base pointers.
+// as offsets from the call's base pointers.
 __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__ members,
                                                   std::uint32_t m, const double* x, double* y,
                                                   double* zp) {
@@ -284,226 +591,41 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
     return M;
 }
 
-template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kUnitWaves) void right_factor_batch(
-    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
-    std::uint32_t count, std::uint32_t total, const double* x, double* y, double* zp) {
-    __shared__ __align__(16) std::uint32_t stage[kUnitWaves][kStageWords + 8];
-    __shared__ real_t<FP32> xs[kUnitWaves][kTile][KB];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const std::uint32_t unit = blockIdx.x * kUnitWaves + wave;
-    if (unit >= total) return;
-    const std::uint32_t m = locate_member(first, count, unit);
-    const MemberView M = bind_member(members, m, x, y, zp);
-    const int p0 = blockIdx.y * KB;
-    if (p0 >= M.k) return;
-    right_factor_unit<KB, FP32>(M, unit - first[m], p0, stage[wave], xs[wave]);
-}
-
-// ---------------------------------------------------------------------------
-// Pass 2: row tiles, kRowWaves waves per tile.
-template <int KB, bool FP32>
-__device__ __forceinline__ void row_tile_unit(const MemberView& M, std::uint32_t tile_index,
-                                              int p0,
-                                              std::uint32_t (*stage_all)[kStageWords + 8],
-                                              double (*partial)[2][64]) {
-    const EntryDev* __restrict__ entries = M.entries;
-    const BlockDev* __restrict__ blocks = M.blocks;
-    const DenseDev* __restrict__ denses = M.denses;
-    const std::int32_t* __restrict__ widths = M.widths;
-    const double* __restrict__ lscale = M.lscale;
-    const double* __restrict__ rscale = M.rscale;
-    const std::uint32_t* __restrict__ words = M.words;
-    const double* __restrict__ dvals = M.dense;
-    const std::int32_t* __restrict__ perm = M.perm;
-    const double* __restrict__ zp = M.zp;
-    const double* __restrict__ x = M.x;
-    double* __restrict__ y = M.y;
-    const std::int64_t ldx = M.ldx, ldy = M.ldy;
-    const int k = M.k;
-
-    // Fields read through the pointer: a local copy indexed by the wave
-    // number would be placed in private (scratch) memory.
-    const TileDev* __restrict__ tile = M.tiles + tile_index;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int kk = min(KB, k - p0);
-    const std::uint32_t R = tile->rows;
-    const bool active = lane < int(R);
-    std::uint32_t* stage = stage_all[wave];
-
-    double acc[KB];
-#pragma unroll
-    for (int p = 0; p < KB; ++p) acc[p] = 0.0;
-
-    const std::uint32_t e_begin = tile->split[wave], e_end = tile->split[wave + 1];
-    for (std::uint32_t e = e_begin; e < e_end; ++e) {
-        const EntryDev ent = entries[e];
-        if (ent.what & kDenseBit) {
-            // Up to kDenseCols columns of a diagonal block (column-major, one
-            // coalesced column per lane-row step): every load is issued before
-            // the first FMA; the x rows are staged in LDS and read as broadcasts.
-            const DenseDev D = denses[ent.what & ~kDenseBit];
-            const std::uint32_t cbeg = ent.words, cnt = ent.cols;
-            double* xsd = reinterpret_cast<double*>(stage);
-            const double* col = dvals + D.off + std::uint64_t(cbeg) * D.m + ent.row + lane;
-            double dv[kDenseCols];
-#pragma unroll
-            for (int i = 0; i < kDenseCols; ++i)
-                dv[i] = (active && i < int(cnt)) ? col[std::uint64_t(i) * D.m] : 0.0;
-            if (lane < int(cnt)) {
-                const double* xr = x + std::int64_t(perm[D.col0 + cbeg + lane]) * ldx + p0;
-#pragma unroll
-                for (int p = 0; p < KB; ++p) xsd[lane * KB + p] = p < kk ? xr[p] : 0.0;
-            }
-            wave_lds_sync();
-#pragma unroll
-            for (int i = 0; i < kDenseCols; ++i) {
-                if (i < int(cnt)) {
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) acc[p] = fma(dv[i], xsd[i * KB + p], acc[p]);
-                }
-            }
-            wave_lds_sync();
-            continue;
-        }
-        const BlockDev B = blocks[ent.what];
-        float blk[KB];
-#pragma unroll
-        for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
-        // Lanes own components: each loads its width, scales and right-factor
-        // sums at once; the component loop then reads them with readlane.
-        // The whole chunk is staged in the same breath when it fits.
-        const bool whole = ent.words + 5 <= std::uint32_t(kStageWords) && B.rank <= 64;
-        std::uint64_t first = ent.chunk;
-        std::uint32_t base = std::uint32_t(first & 3u);
-        for (std::uint32_t j0 = 0; j0 < B.rank;) {
-            const std::uint32_t jn = min(64u, B.rank - j0);
-            const bool mine = lane < int(jn);
-            const std::uint32_t cj = B.comp_base + j0 + lane;
-            const std::uint32_t bits_l = mine ? std::uint32_t(widths[cj]) : 0u;
-            const double s_l = mine ? lscale[cj] * rscale[cj] : 0.0;
-            double z_l[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) z_l[p] = 0.0;
-            if (mine) {
-                for (std::uint32_t sg = 0; sg < B.nseg; ++sg) {
-                    const double* zr = zp + (std::uint64_t(B.zbase) + sg * B.rank + j0 + lane) *
-                                                std::uint64_t(k) + p0;
-#pragma unroll
-                    for (int p = 0; p < KB; ++p)
-                        if (p < kk) z_l[p] += zr[p];
-                }
-            }
-            if (whole) stage_quads(stage, words + (first - base), (ent.words + base + 1 + 3) >> 2,
-                                   lane);
-            const std::uint32_t nw_l = mine ? stream_words(R, bits_l) : 0u;
-            std::uint32_t incl = nw_l;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const std::uint32_t t = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += t;
-            }
-            std::uint32_t take = jn;
-            if (!whole) {  // the components whose streams fit the stage buffer
-                const bool fits = mine && incl + base + 5 <= std::uint32_t(kStageWords);
-                take = std::uint32_t(__popcll(__ballot(fits)));
-            }
-            const std::uint32_t total = __builtin_amdgcn_readlane(incl, take - 1);
-            if (!whole)
-                stage_quads(stage, words + (first - base), (total + base + 1 + 3) >> 2, lane);
-            const std::uint32_t off_l = incl - nw_l;
-            real_t<FP32> zs_l[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) zs_l[p] = real_t<FP32>(s_l * z_l[p]);
-            wave_lds_sync();
-            for (std::uint32_t j = 0; j < take; ++j) {
-                const std::uint32_t bits = __builtin_amdgcn_readlane(bits_l, j);
-                const std::uint32_t off = __builtin_amdgcn_readlane(off_l, j);
-                const int code = active ? decode_code(stage + base + off, lane, bits) : 0;
-                if (!FP32) {
-                    const double cd = double(code);
-#pragma unroll
-                    for (int p = 0; p < KB; ++p)
-                        acc[p] = fma(cd, double(lane_value(zs_l[p], j)), acc[p]);
-                } else if (bits <= 24) {
-                    const float cf = float(code);
-#pragma unroll
-                    for (int p = 0; p < KB; ++p)
-                        blk[p] = fmaf(cf, float(lane_value(zs_l[p], j)), blk[p]);
-                } else {
-                    const float h = float(code >> 16), l = float(code & 0xFFFF);
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) {
-                        const float zf = float(lane_value(zs_l[p], j));
-                        blk[p] = fmaf(h, zf * 65536.0f, blk[p]);
-                        blk[p] = fmaf(l, zf, blk[p]);
-                    }
-                }
-            }
-            first += total;
-            j0 += take;
-            wave_lds_sync();  // the next round (or entry) restages
-            base = std::uint32_t(first & 3u);
-        }
-        if (FP32) {
-#pragma unroll
-            for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
-        }
-    }
-
-    // Add the waves' partial rows in wave order, two right-hand sides at a time.
-    const std::int64_t yrow = active ? std::int64_t(perm[tile->row0 + lane]) * ldy + p0 : 0;
-#pragma unroll
-    for (int pc = 0; pc < KB; pc += 2) {
-        __syncthreads();
-        if (wave > 0) {
-            partial[wave - 1][0][lane] = acc[pc];
-            partial[wave - 1][1][lane] = pc + 1 < KB ? acc[pc + 1] : 0.0;
-        }
-        __syncthreads();
-        if (wave == 0 && active) {
-            double a0 = acc[pc], a1 = pc + 1 < KB ? acc[pc + 1] : 0.0;
-#pragma unroll
-            for (int w = 0; w < kRowWaves - 1; ++w) {
-                a0 += partial[w][0][lane];
-                a1 += partial[w][1][lane];
-            }
-            if (pc < kk) y[yrow + pc] = a0;
-            if (pc + 1 < kk) y[yrow + pc + 1] = a1;
-        }
-    }
-}
-
-template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kRowWaves) void row_tile_pass(MemberView M) {
-    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
-    __shared__ double partial[kRowWaves - 1][2][64];
-    row_tile_unit<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, stage_all, partial);
-}
-
-template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kRowWaves) void row_tile_batch(
-    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
-    std::uint32_t count, const double* x, double* y, double* zp) {
-    __shared__ __align__(16) std::uint32_t stage_all[kRowWaves][kStageWords + 8];
-    __shared__ double partial[kRowWaves - 1][2][64];
+template <int KB, bool FP32, int W>
+__global__ __launch_bounds__(64 * W) void batch_pass1(const MemberView* __restrict__ members,
+                                                      const std::uint32_t* __restrict__ first,
+                                                      std::uint32_t count, const double* x,
+                                                      double* y, double* zp) {
+    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
     const std::uint32_t m = locate_member(first, count, blockIdx.x);
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
-    row_tile_unit<KB, FP32>(M, blockIdx.x - first[m], p0, stage_all, partial);
+    pass1_group<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
 }
 
-// Test hook: codes of one factor side of one block, from its tile chunks,
-// written component-major in the block's own row order.
+template <int KB, bool FP32, int W>
+__global__ __launch_bounds__(64 * W) void batch_pass2(const MemberView* __restrict__ members,
+                                                      const std::uint32_t* __restrict__ first,
+                                                      std::uint32_t count, const double* x,
+                                                      double* y, double* zp) {
+    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
+    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+    const MemberView M = bind_member(members, m, x, y, zp);
+    const int p0 = blockIdx.y * KB;
+    if (p0 >= M.k) return;
+    pass2_tile<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
+}
+
+// Test hook: codes of one factor side of one block, from its chunks, written
+// component-major in the block's own row order.
 __global__ void decode_side(const std::uint32_t* __restrict__ words,
                             const ChunkRef* __restrict__ chunks,
-                            const std::int32_t* __restrict__ widths, std::uint32_t rank,
-                            std::uint32_t rows, std::int32_t* __restrict__ out) {
+                            const std::int32_t* __restrict__ widths, std::uint32_t rows,
+                            std::int32_t* __restrict__ out) {
     const ChunkRef ch = chunks[blockIdx.x];
-    std::uint64_t off = ch.chunk;
-    for (std::uint32_t j = 0; j < rank; ++j) {
+    std::uint64_t off = ch.word;
+    for (std::uint32_t j = ch.j0; j < ch.j0 + ch.jn; ++j) {
         const std::uint32_t bits = widths[j];
         for (std::uint32_t i = threadIdx.x; i < ch.count; i += blockDim.x)
             out[std::uint64_t(j) * rows + ch.first + i] = decode_code(words + off, i, bits);
@@ -511,28 +633,29 @@ __global__ void decode_side(const std::uint32_t* __restrict__ words,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Host side
+
 MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
-                       std::int64_t ldy, double* zp) {
+                       std::int64_t ldy, double* zp, int waves) {
     MemberView M{};
     M.perm = s.perm;
     M.words = s.words;
     M.widths = s.widths;
-    M.lscale = s.lscale;
-    M.rscale = s.rscale;
-    M.dense = s.dense;
-    M.blocks = s.blocks;
-    M.denses = s.denses;
-    M.units = s.units;
+    M.cscale = s.cscale;
+    M.items = s.items;
     M.tiles = s.tiles;
-    M.entries = s.entries;
+    M.rsplit = s.rsplit;
     M.x = x;
     M.y = y;
     M.zp = zp;
     M.ldx = ldx;
     M.ldy = ldy;
     M.k = k;
-    M.units_count = s.n_units;
     M.tiles_count = s.n_tiles;
+    M.rwaves = s.n_rwaves;
+    M.rgroups = (s.n_rwaves + waves - 1) / waves;
+    M.z_rows = s.z_rows;
     return M;
 }
 
@@ -543,41 +666,41 @@ hipEvent_t* profile_slot(const DeviceStore& s) {
     return reinterpret_cast<hipEvent_t*>(p.events.data() + 3 * p.used++);
 }
 
+// Waves per workgroup: fewer waves with more registers each as the
+// right-hand sides per pass grow (the wave lists are cut for 16 waves; a
+// wave of a smaller workgroup takes 16/W consecutive lists).
+constexpr int waves_for(int kb) { return kb >= 16 ? 4 : kb >= 8 ? 8 : 16; }
+
 template <int KB, bool FP32>
 void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
                   std::int64_t ldy, double* zp, hipStream_t stream) {
+    constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((k + KB - 1) / KB);
-    const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
+    const MemberView M = member_view(s, x, ldx, k, y, ldy, zp, W);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
-    if (s.n_units) {
-        hipLaunchKernelGGL((right_factor_pass<KB, FP32>),
-                           dim3((s.n_units + kUnitWaves - 1) / kUnitWaves, passes),
-                           dim3(64 * kUnitWaves), 0, stream, M);
-        hip_check(hipGetLastError(), "right-factor pass launch");
-    }
+    hipLaunchKernelGGL((apply_pass1<KB, FP32, W>), dim3(M.rgroups + s.n_tiles, passes),
+                       dim3(64 * W), 0, stream, M);
+    hip_check(hipGetLastError(), "apply pass 1 launch");
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
-    hipLaunchKernelGGL((row_tile_pass<KB, FP32>), dim3(s.n_tiles, passes), dim3(64 * kRowWaves),
-                       0, stream, M);
-    hip_check(hipGetLastError(), "row-tile pass launch");
+    hipLaunchKernelGGL((apply_pass2<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
+                       stream, M);
+    hip_check(hipGetLastError(), "apply pass 2 launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
 }
 
 template <int KB, bool FP32>
 void launch_batch_class(const BatchClass& c, const double* x, double* y, double* zp,
                         hipStream_t stream) {
+    constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
-    if (c.total_units) {
-        hipLaunchKernelGGL((right_factor_batch<KB, FP32>),
-                           dim3((c.total_units + kUnitWaves - 1) / kUnitWaves, passes),
-                           dim3(64 * kUnitWaves), 0, stream, c.members, c.first_unit, count,
-                           c.total_units, x, y, zp);
-        hip_check(hipGetLastError(), "batched right-factor pass launch");
-    }
-    hipLaunchKernelGGL((row_tile_batch<KB, FP32>), dim3(c.total_tiles, passes),
-                       dim3(64 * kRowWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
-    hip_check(hipGetLastError(), "batched row-tile pass launch");
+    hipLaunchKernelGGL((batch_pass1<KB, FP32, W>), dim3(c.total_groups1, passes), dim3(64 * W),
+                       0, stream, c.members, c.first_group1, count, x, y, zp);
+    hip_check(hipGetLastError(), "batched pass 1 launch");
+    hipLaunchKernelGGL((batch_pass2<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
+                       stream, c.members, c.first_tile, count, x, y, zp);
+    hip_check(hipGetLastError(), "batched pass 2 launch");
 }
 
 template <bool FP32>
@@ -612,9 +735,10 @@ template <class T> T* upload(const std::vector<T>& host, std::uint64_t& bytes) {
     return p;
 }
 
-// Relative cost of a row-tile entry, used to balance the waves of a tile:
-// a fixed latency share plus bytes loaded per lane plus decode work.
-constexpr std::uint64_t kEntryCost = 40;
+// Relative cost of an item for balancing waves: its bytes plus a fixed
+// share for the records, scans and the pipeline turn.
+constexpr std::uint64_t kItemCost = 512;
+std::uint64_t item_cost(const ItemDev& it) { return kItemCost + 16 * std::uint64_t(it.quads); }
 
 // Tiles of at most kTile indices between consecutive cuts.
 std::vector<std::pair<std::uint64_t, std::uint64_t>> make_tiles(std::vector<std::uint64_t> cuts) {
@@ -633,6 +757,23 @@ std::size_t first_tile(const std::vector<std::pair<std::uint64_t, std::uint64_t>
     return std::lower_bound(tiles.begin(), tiles.end(), at,
                             [](const auto& t, std::uint64_t r) { return t.first < r; }) -
            tiles.begin();
+}
+
+// Cuts `cost` into `parts` contiguous runs of similar total; out[i] is the
+// first index of run i (out has parts + 1 entries, offset by `base`).
+void split_runs(const std::vector<std::uint64_t>& cost, std::uint32_t parts, std::uint32_t base,
+                std::uint32_t* out) {
+    std::uint64_t total = 0;
+    for (auto c : cost) total += c;
+    std::uint32_t e = 0;
+    std::uint64_t run = 0;
+    out[0] = base;
+    for (std::uint32_t w = 1; w < parts; ++w) {
+        const std::uint64_t goal = total * w / parts;
+        while (e < cost.size() && run + cost[e] / 2 < goal) run += cost[e++];
+        out[w] = base + e;
+    }
+    out[parts] = base + std::uint32_t(cost.size());
 }
 
 }  // namespace
@@ -691,46 +832,60 @@ Plan build_plan(const qh_store_desc& d) {
     const auto col_tiles = make_tiles(col_cuts);
 
     Plan plan;
-    Plan* s = &plan;
     plan.order = n;
-    std::vector<std::uint32_t> words;
-    std::vector<std::int32_t> widths;
-    std::vector<double> lsc, rsc, dvals;
-    std::vector<BlockDev> blocks;
-    std::vector<DenseDev> denses;
-    std::vector<UnitDev> units;
-    std::vector<std::vector<EntryDev>> per_tile(row_tiles.size());
-    std::vector<std::vector<std::uint64_t>> per_cost(row_tiles.size());
-    s->block_to_lowrank.assign(d.nblocks, -1);
-    std::uint64_t codes = 0, lr_flops = 0, dn_flops = 0, z_rows = 0;
+    std::vector<std::uint32_t>& words = plan.words;
+    std::vector<ItemDev> right_items;
+    std::vector<std::vector<ItemDev>> pass1(row_tiles.size()), pass2(row_tiles.size());
+    plan.block_to_lowrank.assign(d.nblocks, -1);
+    std::uint64_t codes = 0, lr_flops = 0, dn_flops = 0, z_rows = 0, dense_values = 0;
     std::uint64_t n_lowrank_all = 0, left_words = 0, right_words = 0;
+    std::uint64_t pass_words[2] = {0, 0}, fused_components = 0;
 
-    auto pad4 = [&words] {
+    // Starts an item's data on a 16-byte boundary; returns its first quad.
+    auto begin_item = [&words]() {
         while (words.size() & 3u) words.push_back(0u);
+        require(words.size() / 4 < (std::uint64_t{1} << 32), "store too large for one device plan");
+        return std::uint32_t(words.size() / 4);
     };
+    auto finish_item = [&words](ItemDev& it) {
+        const std::uint64_t quads = (words.size() - 4 * std::uint64_t(it.quad) + 3) / 4;
+        if (quads > std::uint64_t(kItemQuads))
+            throw std::logic_error("device plan: item larger than the stage buffer");
+        it.quads = std::uint16_t(quads);
+    };
+
     for (std::uint64_t b = 0; b < d.nblocks; ++b) {
         const std::uint64_t r0 = d.rect[4 * b], r1 = d.rect[4 * b + 1];
         const std::uint64_t c0 = d.rect[4 * b + 2], c1 = d.rect[4 * b + 3];
         const std::uint64_t m = r1 - r0, nc = c1 - c0;
-        const std::size_t t0 = first_tile(row_tiles, r0);
+        const std::size_t rt0 = first_tile(row_tiles, r0), rt1 = first_tile(row_tiles, r1);
+        const std::size_t ct0 = first_tile(col_tiles, c0), ct1 = first_tile(col_tiles, c1);
         if (d.is_dense[b]) {
             require(d.dense && d.dense[b], "dense block without values");
-            const std::uint32_t di = std::uint32_t(denses.size());
-            denses.push_back({std::uint32_t(r0), std::uint32_t(c0), std::uint32_t(m),
-                              std::uint32_t(nc), dvals.size()});
             const double* v = d.dense[b];
-            const std::size_t at = dvals.size();
-            dvals.resize(at + m * nc);
-            for (std::uint64_t i = 0; i < m; ++i)  // row-major in, column-major out
-                for (std::uint64_t c = 0; c < nc; ++c) dvals[at + c * m + i] = v[i * nc + c];
-            for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t)
-                for (std::uint64_t c = 0; c < nc; c += kDenseCols) {
-                    const std::uint64_t cnt = std::min<std::uint64_t>(kDenseCols, nc - c);
-                    per_tile[t].push_back({kDenseBit | di, std::uint32_t(row_tiles[t].first - r0),
-                                           std::uint32_t(c), std::uint32_t(cnt), 0u});
-                    per_cost[t].push_back(kEntryCost + 8 * cnt);  // 8 bytes per lane per column
+            for (std::size_t t = rt0; t < rt1; ++t) {
+                const std::uint64_t a = row_tiles[t].first - r0;
+                const std::uint64_t R = row_tiles[t].second - row_tiles[t].first;
+                for (std::uint64_t c = 0; c < nc; c += kDenseColumns) {
+                    const std::uint64_t cnt = std::min<std::uint64_t>(kDenseColumns, nc - c);
+                    ItemDev it{};
+                    it.quad = begin_item();
+                    it.kind = kDense;
+                    it.count = std::uint8_t(cnt);
+                    it.col = std::uint32_t(c0 + c);
+                    for (std::uint64_t cc = 0; cc < cnt; ++cc)  // row-major in, column-major out
+                        for (std::uint64_t i = 0; i < R; ++i) {
+                            std::uint64_t bitsv;
+                            std::memcpy(&bitsv, v + (a + i) * nc + c + cc, 8);
+                            words.push_back(std::uint32_t(bitsv));
+                            words.push_back(std::uint32_t(bitsv >> 32));
+                        }
+                    finish_item(it);
+                    pass1[t].push_back(it);
                 }
+            }
             dn_flops += 2 * m * nc;
+            dense_values += m * nc;
             continue;
         }
         ++n_lowrank_all;
@@ -740,16 +895,15 @@ Plan build_plan(const qh_store_desc& d) {
         require(d.widths && d.widths[b] && d.lscale && d.lscale[b] && d.rscale && d.rscale[b] &&
                     d.lwords && d.lwords[b] && d.rwords && d.rwords[b],
                 "low-rank block without factors");
-        const std::uint32_t li = std::uint32_t(blocks.size());
-        BlockDev B{std::uint32_t(r), std::uint32_t(widths.size()), std::uint32_t(z_rows), 0};
+        const std::int64_t li = std::int64_t(plan.ranks.size());
+        const std::uint32_t comp_base = std::uint32_t(plan.widths.size());
         std::vector<std::uint64_t> loff(r), roff(r), lcnt(r), rcnt(r);
         std::uint64_t lw = 0, rw = 0;
         for (std::uint64_t j = 0; j < r; ++j) {
             const int bits = d.widths[b][j];
             require(bits >= 2 && bits <= 32, "bit width must be in [2, 32]");
-            widths.push_back(bits);
-            lsc.push_back(d.lscale[b][j]);
-            rsc.push_back(d.rscale[b][j]);
+            plan.widths.push_back(bits);
+            plan.cscale.push_back(d.lscale[b][j] * d.rscale[b][j]);
             loff[j] = lw;
             roff[j] = rw;
             lcnt[j] = (m * bits + 31) / 32;
@@ -759,103 +913,185 @@ Plan build_plan(const qh_store_desc& d) {
         }
         left_words += lw;
         right_words += rw;
-        // left factor: one chunk per row tile
-        s->left_chunks.emplace_back();
-        for (std::size_t t = t0; t < row_tiles.size() && row_tiles[t].first < r1; ++t) {
-            const std::uint64_t a = row_tiles[t].first - r0;
-            const std::uint64_t rows = row_tiles[t].second - row_tiles[t].first;
-            const std::uint64_t at = words.size();
-            for (std::uint64_t j = 0; j < r; ++j)
-                append_codes(d.lwords[b] + loff[j], lcnt[j], a, rows, d.widths[b][j], words);
-            const std::uint64_t chunk_words = words.size() - at;
-            pad4();
-            s->left_chunks.back().push_back({std::uint32_t(a), std::uint32_t(rows), at});
-            require(chunk_words + 8 < (std::uint64_t{1} << 32), "block chunk too large");
-            per_tile[t].push_back({li, std::uint32_t(a), std::uint32_t(chunk_words), 0u, at});
-            per_cost[t].push_back(kEntryCost + 4 * r + chunk_words / 16);  // bytes per lane + decode
+        plan.left_chunks.emplace_back();
+        plan.right_chunks.emplace_back();
+        auto& lch = plan.left_chunks.back();
+        auto& rch = plan.right_chunks.back();
+        const int* wd = d.widths[b];
+        // Components [j0, j) whose streams (per component `per(j)` words) fit
+        // one item with at most `cap` components.
+        auto group_end = [&](std::uint64_t j0, std::uint64_t cap, auto per) {
+            std::uint64_t j = j0, w = 0;
+            while (j < r && j - j0 < cap) {
+                const std::uint64_t add = per(j);
+                if ((w + add + 3) / 4 > std::uint64_t(kItemQuads)) break;
+                w += add;
+                ++j;
+            }
+            if (j == j0) throw std::logic_error("device plan: component stream exceeds an item");
+            return j;
+        };
+
+        if (rt1 - rt0 == 1 && ct1 - ct0 == 1) {
+            // One row tile and one column tile: both factors in one item.
+            const std::uint64_t R = m, Cn = nc;
+            for (std::uint64_t j0 = 0; j0 < r;) {
+                const std::uint64_t j1 = group_end(j0, kItemComponents, [&](std::uint64_t j) {
+                    return std::uint64_t(stream_words(std::uint32_t(Cn), wd[j]) +
+                                         stream_words(std::uint32_t(R), wd[j]));
+                });
+                ItemDev it{};
+                it.quad = begin_item();
+                it.kind = kFused;
+                it.count = std::uint8_t(j1 - j0);
+                it.comp0 = comp_base + std::uint32_t(j0);
+                it.col = std::uint32_t(c0);
+                it.cols = std::uint32_t(Cn);
+                rch.push_back({0u, std::uint32_t(Cn), std::uint32_t(j0), std::uint32_t(j1 - j0),
+                               words.size()});
+                for (std::uint64_t j = j0; j < j1; ++j)
+                    append_codes(d.rwords[b] + roff[j], rcnt[j], 0, Cn, wd[j], words);
+                lch.push_back({0u, std::uint32_t(R), std::uint32_t(j0), std::uint32_t(j1 - j0),
+                               words.size()});
+                for (std::uint64_t j = j0; j < j1; ++j)
+                    append_codes(d.lwords[b] + loff[j], lcnt[j], 0, R, wd[j], words);
+                finish_item(it);
+                pass1[rt0].push_back(it);
+                j0 = j1;
+            }
+            pass_words[0] += lw + rw;
+            fused_components += r;
+        } else {
+            // Column tiles in runs of at most 64: each run has its own partial
+            // sums, read by the run's kLeft items (64 pairs per item).
+            for (std::size_t v0 = ct0; v0 < ct1; v0 += 64) {
+                const std::size_t v1 = std::min(ct1, v0 + 64);
+                const std::uint64_t nseg = v1 - v0;
+                const std::uint64_t zbase = z_rows;
+                for (std::size_t t = v0; t < v1; ++t) {
+                    const std::uint64_t a = col_tiles[t].first - c0;
+                    const std::uint64_t Cn = col_tiles[t].second - col_tiles[t].first;
+                    for (std::uint64_t j0 = 0; j0 < r;) {
+                        const std::uint64_t j1 = group_end(j0, kItemComponents, [&](std::uint64_t j) {
+                            return std::uint64_t(stream_words(std::uint32_t(Cn), wd[j]));
+                        });
+                        ItemDev it{};
+                        it.quad = begin_item();
+                        it.kind = kRight;
+                        it.count = std::uint8_t(j1 - j0);
+                        it.comp0 = comp_base + std::uint32_t(j0);
+                        it.col = std::uint32_t(col_tiles[t].first);
+                        it.cols = std::uint32_t(Cn);
+                        it.zrow = std::uint32_t(zbase + (t - v0) * r + j0);
+                        rch.push_back({std::uint32_t(a), std::uint32_t(Cn), std::uint32_t(j0),
+                                       std::uint32_t(j1 - j0), words.size()});
+                        for (std::uint64_t j = j0; j < j1; ++j)
+                            append_codes(d.rwords[b] + roff[j], rcnt[j], a, Cn, wd[j], words);
+                        finish_item(it);
+                        right_items.push_back(it);
+                        j0 = j1;
+                    }
+                }
+                z_rows += nseg * r;
+                require(z_rows < (std::uint64_t{1} << 31), "store too large for one device plan");
+                for (std::size_t t = rt0; t < rt1; ++t) {
+                    const std::uint64_t a = row_tiles[t].first - r0;
+                    const std::uint64_t R = row_tiles[t].second - row_tiles[t].first;
+                    for (std::uint64_t j0 = 0; j0 < r;) {
+                        const std::uint64_t j1 = group_end(j0, std::max<std::uint64_t>(1, 64 / nseg),
+                                                           [&](std::uint64_t j) {
+                            return std::uint64_t(stream_words(std::uint32_t(R), wd[j]));
+                        });
+                        ItemDev it{};
+                        it.quad = begin_item();
+                        it.kind = kLeft;
+                        it.count = std::uint8_t(j1 - j0);
+                        it.comp0 = comp_base + std::uint32_t(j0);
+                        it.zrow = std::uint32_t(zbase + j0);
+                        it.zstride = std::uint32_t(r);
+                        it.nseg = std::uint32_t(nseg);
+                        if (v0 == ct0)
+                            lch.push_back({std::uint32_t(a), std::uint32_t(R), std::uint32_t(j0),
+                                           std::uint32_t(j1 - j0), words.size()});
+                        for (std::uint64_t j = j0; j < j1; ++j)
+                            append_codes(d.lwords[b] + loff[j], lcnt[j], a, R, wd[j], words);
+                        finish_item(it);
+                        pass2[t].push_back(it);
+                        j0 = j1;
+                    }
+                }
+            }
+            pass_words[0] += rw;
+            pass_words[1] += lw;
         }
-        // right factor: one chunk (and one work unit) per column tile
-        s->right_chunks.emplace_back();
-        for (std::size_t t = first_tile(col_tiles, c0);
-             t < col_tiles.size() && col_tiles[t].first < c1; ++t) {
-            const std::uint64_t a = col_tiles[t].first - c0;
-            const std::uint64_t cols = col_tiles[t].second - col_tiles[t].first;
-            const std::uint64_t at = words.size();
-            for (std::uint64_t j = 0; j < r; ++j)
-                append_codes(d.rwords[b] + roff[j], rcnt[j], a, cols, d.widths[b][j], words);
-            const std::uint64_t chunk_words = words.size() - at;
-            require(chunk_words + 8 < (std::uint64_t{1} << 32), "block chunk too large");
-            pad4();
-            s->right_chunks.back().push_back({std::uint32_t(a), std::uint32_t(cols), at});
-            units.push_back({li, std::uint32_t(col_tiles[t].first), std::uint32_t(cols),
-                             std::uint32_t(z_rows + B.nseg * r), std::uint32_t(chunk_words), 0u,
-                             at});
-            ++B.nseg;
-        }
-        z_rows += std::uint64_t(B.nseg) * r;
-        s->block_to_lowrank[b] = li;
-        blocks.push_back(B);
+        plan.block_to_lowrank[b] = li;
+        plan.comp_base.push_back(comp_base);
+        plan.ranks.push_back(std::uint32_t(r));
         codes += (m + nc) * r;
         lr_flops += 2 * r * (m + nc);
     }
-    require(z_rows < (std::uint64_t{1} << 31), "store too large for one device plan");
-    const std::uint64_t stored_words = words.size();
-    words.resize(words.size() + 8, 0u);  // reads one word past the last chunk
+    plan.stored_words = words.size();
+    words.resize(words.size() + 8, 0u);  // one readable word past the last item
 
-    std::vector<TileDev> tiles(row_tiles.size());
-    std::vector<EntryDev> entries;
+    // Item table: right-factor items, then per tile its pass-1 and pass-2 items.
+    std::vector<ItemDev>& items = plan.items;
+    items = right_items;
+    plan.tiles.resize(row_tiles.size());
+    std::uint64_t tile_cost = 0;
     for (std::size_t t = 0; t < row_tiles.size(); ++t) {
-        TileDev& T = tiles[t];
+        TileDev& T = plan.tiles[t];
         T.row0 = std::uint32_t(row_tiles[t].first);
         T.rows = std::uint32_t(row_tiles[t].second - row_tiles[t].first);
-        const auto& list = per_tile[t];
-        const auto& cost = per_cost[t];
-        const std::uint32_t base = std::uint32_t(entries.size());
-        entries.insert(entries.end(), list.begin(), list.end());
-        // Contiguous split of the entries into kRowWaves runs of similar cost.
-        std::uint64_t total = 0;
-        for (auto c : cost) total += c;
-        std::uint32_t e = 0;
-        std::uint64_t run = 0;
-        T.split[0] = base;
-        for (int w = 1; w < kRowWaves; ++w) {
-            const std::uint64_t goal = total * std::uint64_t(w) / kRowWaves;
-            while (e < list.size() && run + cost[e] / 2 < goal) run += cost[e++];
-            T.split[w] = base + e;
+        for (int pass = 0; pass < 2; ++pass) {
+            const auto& list = pass == 0 ? pass1[t] : pass2[t];
+            std::vector<std::uint64_t> cost;
+            for (const auto& it : list) cost.push_back(item_cost(it));
+            if (pass == 0)
+                for (auto c : cost) tile_cost += c;
+            split_runs(cost, kWaves, std::uint32_t(items.size()), pass == 0 ? T.split1 : T.split2);
+            items.insert(items.end(), list.begin(), list.end());
         }
-        T.split[kRowWaves] = base + std::uint32_t(list.size());
     }
+    require(items.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
+    // Right waves carry about what a pass-1 tile wave carries.
+    std::vector<std::uint64_t> rcost;
+    std::uint64_t rtotal = 0;
+    for (const auto& it : right_items) {
+        rcost.push_back(item_cost(it));
+        rtotal += rcost.back();
+    }
+    if (!right_items.empty()) {
+        const std::uint64_t target = std::max<std::uint64_t>(
+            8 * kItemCost, tile_cost / std::max<std::uint64_t>(1, kWaves * row_tiles.size()));
+        const std::uint64_t waves =
+            std::min<std::uint64_t>(right_items.size(), (rtotal + target - 1) / target);
+        plan.rsplit.resize(waves + 1);
+        split_runs(rcost, std::uint32_t(waves), 0, plan.rsplit.data());
+    }
+    plan.z_rows = z_rows;
 
     qh_store_stats& st = plan.stats;
     st.order = n;
     st.nblocks = d.nblocks;
-    st.ndense = denses.size();
+    std::uint64_t ndense = 0;
+    for (std::uint64_t b = 0; b < d.nblocks; ++b) ndense += d.is_dense[b] ? 1 : 0;
+    st.ndense = ndense;
     st.nlowrank = n_lowrank_all;
-    st.ncomponents = widths.size();
+    st.ncomponents = plan.widths.size();
     st.code_words = left_words + right_words;
     st.left_code_words = left_words;
     st.right_code_words = right_words;
     st.codes = codes;
-    st.dense_values = dvals.size();
+    st.dense_values = dense_values;
     st.lowrank_flops_per_rhs = lr_flops;
     st.dense_flops_per_rhs = dn_flops;
-    st.tiles = tiles.size();
-    st.items = units.size();
-    st.entries = entries.size();
-
+    st.tiles = plan.tiles.size();
+    st.items = items.size();
+    st.right_waves = plan.rsplit.empty() ? 0 : plan.rsplit.size() - 1;
+    st.pass_code_words[0] = pass_words[0];
+    st.pass_code_words[1] = pass_words[1];
+    st.fused_components = fused_components;
     plan.perm = std::move(perm);
-    plan.words = std::move(words);
-    plan.widths = std::move(widths);
-    plan.lscale = std::move(lsc);
-    plan.rscale = std::move(rsc);
-    plan.dense = std::move(dvals);
-    plan.blocks = std::move(blocks);
-    plan.denses = std::move(denses);
-    plan.units = std::move(units);
-    plan.tiles = std::move(tiles);
-    plan.entries = std::move(entries);
-    plan.z_rows = z_rows;
-    plan.stored_words = stored_words;
     return plan;
 }
 
@@ -873,14 +1109,10 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
         s->perm = upload(plan.perm, bytes);
         s->words = upload(plan.words, bytes);
         s->widths = upload(plan.widths, bytes);
-        s->lscale = upload(plan.lscale, bytes);
-        s->rscale = upload(plan.rscale, bytes);
-        s->dense = upload(plan.dense, bytes);
-        s->blocks = upload(plan.blocks, bytes);
-        s->denses = upload(plan.denses, bytes);
-        s->units = upload(plan.units, bytes);
+        s->cscale = upload(plan.cscale, bytes);
+        s->items = upload(plan.items, bytes);
         s->tiles = upload(plan.tiles, bytes);
-        s->entries = upload(plan.entries, bytes);
+        s->rsplit = upload(plan.rsplit, bytes);
         s->device_bytes = bytes;
     } catch (...) {
         destroy_device_store(s);
@@ -888,20 +1120,15 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     }
     s->n_words = plan.stored_words;
     s->n_comp = plan.widths.size();
-    s->n_dense_values = plan.dense.size();
-    s->n_lowrank = std::uint32_t(plan.blocks.size());
-    s->n_dense = std::uint32_t(plan.denses.size());
-    s->n_units = std::uint32_t(plan.units.size());
     s->n_tiles = std::uint32_t(plan.tiles.size());
-    s->n_entries = std::uint32_t(plan.entries.size());
+    s->n_items = std::uint32_t(plan.items.size());
+    s->n_rwaves = plan.rsplit.empty() ? 0u : std::uint32_t(plan.rsplit.size() - 1);
     s->z_rows = plan.z_rows;
     s->block_to_lowrank = std::move(plan.block_to_lowrank);
     s->left_chunks = std::move(plan.left_chunks);
     s->right_chunks = std::move(plan.right_chunks);
-    for (const auto& B : plan.blocks) {
-        s->comp_base.push_back(B.comp_base);
-        s->ranks.push_back(B.rank);
-    }
+    s->comp_base = std::move(plan.comp_base);
+    s->ranks = std::move(plan.ranks);
     s->stats = plan.stats;
     s->stats.device_bytes = s->device_bytes;
     s->stats.device = device;
@@ -916,94 +1143,133 @@ template <class V> const typename V::value_type& at(const V& v, std::uint64_t i)
 
 void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
     const std::uint64_t n = pl.order;
-    std::vector<double> zp(pl.z_rows * std::uint64_t(k), 0.0);
-    auto code_at = [&](std::uint64_t seg_word, std::uint32_t idx, std::uint32_t bits) {
-        const std::uint32_t pos = idx * bits;
-        const std::uint64_t w = seg_word + (pos >> 5);
-        const std::uint64_t pair = std::uint64_t(at(pl.words, w)) |
-                                   (std::uint64_t(at(pl.words, w + 1)) << 32);
+    const std::uint64_t K = std::uint64_t(k);
+    std::vector<double> zp((pl.z_rows + n) * K, 0.0);
+    auto check = [](bool ok, const char* what) {
+        if (!ok) throw std::out_of_range(what);
+    };
+    // Word `w` of an item's data: inside what the wave stages, plus the one
+    // word past the data that a decode may read.
+    auto word = [&](const ItemDev& it, std::uint64_t w) {
+        check(w < 4 * std::uint64_t(kItemQuads + 1), "stage read out of range");
+        return at(pl.words, 4 * std::uint64_t(it.quad) + w);
+    };
+    auto code_at = [&](const ItemDev& it, std::uint64_t off, std::uint32_t idx, std::uint32_t bits) {
+        const std::uint64_t pos = std::uint64_t(idx) * bits;
+        check(32 * off + pos + bits <= 128 * std::uint64_t(it.quads), "code outside the item data");
+        const std::uint64_t w = off + (pos >> 5);
+        const std::uint64_t pair = std::uint64_t(word(it, w)) | (std::uint64_t(word(it, w + 1)) << 32);
         const std::uint64_t raw = (pair >> (pos & 31u)) & ((std::uint64_t{1} << bits) - 1);
         const std::uint64_t sign = std::uint64_t{1} << (bits - 1);
         return double(std::int64_t(raw ^ sign) - std::int64_t(sign));
     };
-    // Pass 1: every unit, the components in passes of kPassComponents.
-    for (const UnitDev& u : pl.units) {
-        const BlockDev& B = at(pl.blocks, u.block);
-        if (u.cols == 0 || u.cols > std::uint32_t(kTile)) throw std::out_of_range("unit width");
-        std::uint64_t all = 0;
-        for (std::uint32_t j = 0; j < B.rank; ++j)
-            all += stream_words(u.cols, at(pl.widths, B.comp_base + j));
-        if (all != u.words) throw std::out_of_range("unit word count");
-        if (u.words + 5 <= std::uint32_t(kStageWords))  // staged whole, one word past the end
-            at(pl.words, (u.chunk & ~std::uint64_t(3)) + 4 * ((u.words + (u.chunk & 3) + 4) >> 2) - 1);
-        std::uint64_t done = 0;
-        for (std::uint32_t j0 = 0; j0 < B.rank; j0 += kPassComponents) {
-            const std::uint32_t jn = std::min<std::uint32_t>(kPassComponents, B.rank - j0);
-            std::uint64_t total = 0;
-            for (std::uint32_t g = 0; g < jn; ++g)
-                total += stream_words(u.cols, at(pl.widths, B.comp_base + j0 + g));
-            const std::uint64_t first = u.chunk + done;
-            const std::uint64_t delta = first & 3u;
-            const std::uint64_t quads = (total + delta + 1 + 3) >> 2;
-            if (4 * quads > std::uint64_t(kStageWords + 8)) throw std::out_of_range("stage size");
-            at(pl.words, first - delta + 4 * quads - 1);  // staged range is readable
-            std::uint64_t seg = 0;
-            for (std::uint32_t g = 0; g < jn; ++g) {
-                const std::uint32_t j = j0 + g;
-                const std::uint32_t bits = at(pl.widths, B.comp_base + j);
-                for (std::int64_t p = 0; p < k; ++p) {
-                    double sum = 0.0;
-                    for (std::uint32_t c = 0; c < u.cols; ++c)
-                        sum += code_at(first + seg, c, bits) *
-                               x[std::int64_t(at(pl.perm, u.col0 + c)) * k + p];
-                    zp.at((std::uint64_t(u.zrow) + j) * std::uint64_t(k) + p) = sum;
-                }
-                seg += stream_words(u.cols, bits);
+    auto check_item = [&](const ItemDev& it) {
+        check(it.quads <= kItemQuads, "item size");
+        check(it.count >= 1 && it.count <= kItemComponents, "item component count");
+        check(4 * (std::uint64_t(it.quad) + it.quads) <= pl.words.size(), "item data range");
+    };
+    auto xval = [&](std::uint32_t col, std::uint64_t p) {
+        return x[std::uint64_t(at(pl.perm, col)) * K + p];
+    };
+    // Right-factor sums of components [0, count) of an item whose streams
+    // start at word 0: z[g][p].
+    auto right_sums = [&](const ItemDev& it, std::vector<double>& z) {
+        check(it.cols >= 1 && it.cols <= std::uint32_t(kTile), "item columns");
+        z.assign(std::uint64_t(it.count) * K, 0.0);
+        std::uint64_t off = 0;
+        for (std::uint32_t g = 0; g < it.count; ++g) {
+            const std::uint32_t bits = at(pl.widths, it.comp0 + g);
+            for (std::uint64_t p = 0; p < K; ++p) {
+                double s = 0.0;
+                for (std::uint32_t c = 0; c < it.cols; ++c)
+                    s += code_at(it, off, c, bits) * xval(it.col + c, p);
+                z[g * K + p] = s;
             }
-            done += total;
+            off += stream_words(it.cols, bits);
         }
-    }
-    // Pass 2: every tile, every wave's entries.
-    for (const TileDev& T : pl.tiles) {
-        if (T.rows == 0 || T.rows > std::uint32_t(kTile)) throw std::out_of_range("tile rows");
-        for (std::int64_t p = 0; p < k; ++p)
-            for (std::uint32_t i = 0; i < T.rows; ++i) {
-                double acc = 0.0;
-                for (int w = 0; w < kRowWaves; ++w)
-                    for (std::uint32_t e = T.split[w]; e < T.split[w + 1]; ++e) {
-                        const EntryDev& ent = at(pl.entries, e);
-                        if (ent.what & kDenseBit) {
-                            const DenseDev& D = at(pl.denses, ent.what & ~kDenseBit);
-                            if (ent.cols == 0 || ent.cols > std::uint32_t(kDenseCols) ||
-                                ent.words + ent.cols > D.n)
-                                throw std::out_of_range("dense entry columns");
-                            for (std::uint32_t c = ent.words; c < ent.words + ent.cols; ++c)
-                                acc += at(pl.dense, D.off + std::uint64_t(c) * D.m + ent.row + i) *
-                                       x[std::int64_t(at(pl.perm, D.col0 + c)) * k + p];
-                            continue;
-                        }
-                        const BlockDev& B = at(pl.blocks, ent.what);
-                        std::uint64_t all = 0;
-                        for (std::uint32_t j = 0; j < B.rank; ++j)
-                            all += stream_words(T.rows, at(pl.widths, B.comp_base + j));
-                        if (all != ent.words) throw std::out_of_range("entry word count");
-                        at(pl.words, (ent.chunk & ~std::uint64_t(3)) +
-                                         4 * ((ent.words + (ent.chunk & 3) + 4) >> 2) - 1);
-                        std::uint64_t off = ent.chunk;
-                        for (std::uint32_t j = 0; j < B.rank; ++j) {
-                            const std::uint32_t cj = B.comp_base + j;
-                            const std::uint32_t bits = at(pl.widths, cj);
-                            double z = 0.0;
-                            for (std::uint32_t sg = 0; sg < B.nseg; ++sg)
-                                z += zp.at((std::uint64_t(B.zbase) + sg * B.rank + j) * k + p);
-                            acc += code_at(off, i, bits) * (at(pl.lscale, cj) * at(pl.rscale, cj) * z);
-                            off += stream_words(T.rows, bits);
-                        }
+        return off;
+    };
+    // Left-factor rows: acc[i][p] += sum_g code(i, g) * cscale_g * z[g][p].
+    auto left_rows = [&](const ItemDev& it, std::uint64_t off, std::uint32_t R,
+                         const std::vector<double>& z, std::vector<double>& acc) {
+        for (std::uint32_t g = 0; g < it.count; ++g) {
+            const std::uint32_t bits = at(pl.widths, it.comp0 + g);
+            const double sc = at(pl.cscale, it.comp0 + g);
+            for (std::uint32_t i = 0; i < R; ++i)
+                for (std::uint64_t p = 0; p < K; ++p)
+                    acc[i * K + p] += code_at(it, off, i, bits) * (sc * z[g * K + p]);
+            off += stream_words(R, bits);
+        }
+    };
+    std::vector<double> z;
+    // Pass 1, right waves.
+    for (std::size_t w = 0; w + 1 < pl.rsplit.size(); ++w)
+        for (std::uint32_t i = at(pl.rsplit, w); i < at(pl.rsplit, w + 1); ++i) {
+            const ItemDev& it = at(pl.items, i);
+            check_item(it);
+            check(it.kind == kRight, "right wave holds another item kind");
+            right_sums(it, z);
+            for (std::uint32_t g = 0; g < it.count; ++g)
+                for (std::uint64_t p = 0; p < K; ++p)
+                    zp.at((std::uint64_t(it.zrow) + g) * K + p) = z[g * K + p];
+        }
+    // Pass 1 and pass 2 tiles, waves added in order.
+    for (int pass = 0; pass < 2; ++pass)
+        for (const TileDev& T : pl.tiles) {
+            const std::uint32_t R = T.rows;
+            check(R >= 1 && R <= std::uint32_t(kTile), "tile rows");
+            const std::uint32_t* split = pass == 0 ? T.split1 : T.split2;
+            std::vector<double> total(std::uint64_t(R) * K, 0.0);
+            if (pass == 1)
+                for (std::uint32_t i = 0; i < R; ++i)
+                    for (std::uint64_t p = 0; p < K; ++p)
+                        total[i * K + p] = zp.at((pl.z_rows + T.row0 + i) * K + p);
+            for (int w = 0; w < kWaves; ++w) {
+                std::vector<double> acc(std::uint64_t(R) * K, 0.0);
+                for (std::uint32_t e = split[w]; e < split[w + 1]; ++e) {
+                    const ItemDev& it = at(pl.items, e);
+                    check_item(it);
+                    if (pass == 0 && it.kind == kDense) {
+                        check(it.count <= kDenseColumns, "dense item columns");
+                        for (std::uint32_t c = 0; c < it.count; ++c)
+                            for (std::uint32_t i = 0; i < R; ++i) {
+                                const std::uint64_t wv = 2 * (std::uint64_t(c) * R + i);
+                                const std::uint64_t bitsv =
+                                    std::uint64_t(word(it, wv)) | (std::uint64_t(word(it, wv + 1)) << 32);
+                                check(wv + 2 <= 4 * std::uint64_t(it.quads), "dense value outside the item");
+                                double v;
+                                std::memcpy(&v, &bitsv, 8);
+                                for (std::uint64_t p = 0; p < K; ++p)
+                                    acc[i * K + p] += v * xval(it.col + c, p);
+                            }
+                    } else if (pass == 0 && it.kind == kFused) {
+                        const std::uint64_t roff = right_sums(it, z);
+                        left_rows(it, roff, R, z, acc);
+                    } else if (pass == 1 && it.kind == kLeft) {
+                        check(std::uint64_t(it.count) * it.nseg <= 64 && it.nseg >= 1, "left item pairs");
+                        z.assign(std::uint64_t(it.count) * K, 0.0);
+                        for (std::uint32_t g = 0; g < it.count; ++g)
+                            for (std::uint32_t s = 0; s < it.nseg; ++s)
+                                for (std::uint64_t p = 0; p < K; ++p)
+                                    z[g * K + p] += zp.at((std::uint64_t(it.zrow) +
+                                                           std::uint64_t(s) * it.zstride + g) * K + p);
+                        check(std::uint64_t(it.zrow) + std::uint64_t(it.nseg - 1) * it.zstride +
+                                  it.count <= pl.z_rows, "left item reads past its partial sums");
+                        left_rows(it, 0, R, z, acc);
+                    } else {
+                        throw std::out_of_range("item kind in the wrong pass");
                     }
-                y[std::int64_t(at(pl.perm, T.row0 + i)) * k + p] = acc;
+                }
+                for (std::size_t i = 0; i < total.size(); ++i) total[i] += acc[i];
             }
-    }
-    (void)n;
+            for (std::uint32_t i = 0; i < R; ++i)
+                for (std::uint64_t p = 0; p < K; ++p) {
+                    if (pass == 0)
+                        zp.at((pl.z_rows + T.row0 + i) * K + p) = total[i * K + p];
+                    else
+                        y[std::uint64_t(at(pl.perm, T.row0 + i)) * K + p] = total[i * K + p];
+                }
+        }
 }
 
 void destroy_device_store(DeviceStore* s) {
@@ -1013,18 +1279,16 @@ void destroy_device_store(DeviceStore* s) {
     (void)hipSetDevice(s->device);
     for (void* e : s->profile.events) (void)hipEventDestroy(static_cast<hipEvent_t>(e));
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
-                    static_cast<void*>(s->widths), static_cast<void*>(s->lscale),
-                    static_cast<void*>(s->rscale), static_cast<void*>(s->dense),
-                    static_cast<void*>(s->blocks), static_cast<void*>(s->denses),
-                    static_cast<void*>(s->units), static_cast<void*>(s->tiles),
-                    static_cast<void*>(s->entries)})
+                    static_cast<void*>(s->widths), static_cast<void*>(s->cscale),
+                    static_cast<void*>(s->items), static_cast<void*>(s->tiles),
+                    static_cast<void*>(s->rsplit)})
         if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete s;
 }
 
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k) {
-    return s.z_rows * std::uint64_t(k) * sizeof(double);
+    return (s.z_rows + s.order) * std::uint64_t(k) * sizeof(double);
 }
 
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,
@@ -1098,6 +1362,7 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
     const std::int64_t li = s.block_to_lowrank[block];
     if (li < 0) return;  // dense or rank 0: nothing to decode
     const auto& chunks = side == 0 ? s.left_chunks[li] : s.right_chunks[li];
+    if (chunks.empty()) return;
     std::uint32_t rows = 0;
     for (const auto& c : chunks) rows = std::max(rows, c.first + c.count);
     DeviceScope scope(s.device);
@@ -1109,7 +1374,7 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
                              hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(decode_side, dim3(unsigned(chunks.size())), dim3(64), 0, st, s.words,
-                           dev, s.widths + s.comp_base[li], s.ranks[li], rows, out);
+                           dev, s.widths + s.comp_base[li], rows, out);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
@@ -1142,7 +1407,7 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
             b->offset.push_back(off);
             b->zoffset.push_back(zoff);
             off += stores[i]->order * std::uint64_t(k[i]);
-            zoff += stores[i]->z_rows * std::uint64_t(k[i]);
+            zoff += apply_workspace_bytes(*stores[i], k[i]) / sizeof(double);
         }
         b->total = off;
         b->z_total = zoff;
@@ -1150,8 +1415,9 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
         for (int kb : {1, 2, 4, 8, 16}) {
             BatchClass c;
             c.kb = kb;
+            const int W = waves_for(kb);
             std::vector<MemberView> views;
-            std::vector<std::uint32_t> fu{0}, ft{0};
+            std::vector<std::uint32_t> fg{0}, ft{0};
             for (std::uint64_t i = 0; i < count; ++i) {
                 if (rhs_class(b->k[i]) != kb) continue;
                 const DeviceStore& s = *b->stores[i];
@@ -1159,21 +1425,22 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
                 views.push_back(member_view(
                     s, reinterpret_cast<const double*>(std::uintptr_t(b->offset[i])), b->k[i],
                     int(b->k[i]), reinterpret_cast<double*>(std::uintptr_t(b->offset[i])),
-                    b->k[i], reinterpret_cast<double*>(std::uintptr_t(b->zoffset[i]))));
-                require(std::uint64_t(fu.back()) + s.n_units < (std::uint64_t{1} << 31) &&
+                    b->k[i], reinterpret_cast<double*>(std::uintptr_t(b->zoffset[i])), W));
+                const std::uint64_t groups = views.back().rgroups + std::uint64_t(s.n_tiles);
+                require(std::uint64_t(fg.back()) + groups < (std::uint64_t{1} << 31) &&
                             std::uint64_t(ft.back()) + s.n_tiles < (std::uint64_t{1} << 31),
                         "batch too large for one launch");
-                fu.push_back(fu.back() + s.n_units);
+                fg.push_back(fg.back() + std::uint32_t(groups));
                 ft.push_back(ft.back() + s.n_tiles);
                 c.max_k = std::max(c.max_k, b->k[i]);
             }
             if (views.empty()) continue;
             c.member_count = views.size();
-            c.total_units = fu.back();
+            c.total_groups1 = fg.back();
             c.total_tiles = ft.back();
             std::uint64_t bytes = 0;
             c.members = upload(views, bytes);
-            c.first_unit = upload(fu, bytes);
+            c.first_group1 = upload(fg, bytes);
             c.first_tile = upload(ft, bytes);
             b->classes.push_back(c);
         }
@@ -1190,7 +1457,7 @@ void destroy_batch(Batch* b) {
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(b->device);
     for (auto& c : b->classes)
-        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_unit),
+        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_group1),
                         static_cast<void*>(c.first_tile)})
             if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);

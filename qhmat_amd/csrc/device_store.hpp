@@ -1,15 +1,37 @@
 // Device-resident packed store and the apply plan (internal to the library).
 //
-// Layout in HBM ("tile-major"): the matrix rows are cut into row tiles of at
-// most kTile rows that never cross a block boundary, and the columns into
-// column tiles the same way.  Each low-rank block's left factor is stored as
-// one chunk per row tile it covers and its right factor as one chunk per
-// column tile: a chunk holds, for every component j, the codes of that
-// tile's rows (or columns) as a word-aligned LSB-first stream of
-// ceil(R*b_j/32) words -- the reference's packed format (core.cpp:235-267)
-// cut at tile boundaries -- and is padded to a multiple of four words so
-// that a wave can stream it with 16-byte loads.  When a tile starts on a
-// multiple of 32 codes the chunk's words are the reference's words verbatim.
+// Layout in HBM.  The matrix rows are cut into row tiles of at most kTile
+// rows that never cross a block boundary, and the columns into column tiles
+// the same way.  All work of an apply is a list of *items*; each item's data
+// (codes or diagonal values) is one contiguous, 16-byte aligned run of at
+// most kItemQuads 16-byte groups in the `words` array, so that a wave can
+// request an item with a fixed number of 16-byte loads per lane before it
+// needs any of it.  Item kinds:
+//
+//  kDense  columns [col, col+count) of a diagonal block restricted to one
+//          row tile: R x count FP64 values, column-major (R = tile rows).
+//  kFused  a low-rank block that lies in one row tile and one column tile,
+//          components [comp0, comp0+count): first the right-factor codes of
+//          those components (one word-aligned stream of ceil(cols*b/32)
+//          words each), then their left-factor codes (ceil(R*b/32) words
+//          each).  The apply forms z = V^T x and y += U z inside the item.
+//  kRight  one column tile of a larger low-rank block, components
+//          [comp0, comp0+count): right-factor streams only.  Writes the
+//          partial sums of its column tile to the workspace.
+//  kLeft   one row tile of a larger low-rank block, components
+//          [comp0, comp0+count): left-factor streams only.  Reads the
+//          partial sums of the block's column tiles from the workspace.
+//
+// A stream is the reference's packed format (core.cpp:235-267) restricted to
+// the tile's rows or columns: codes LSB-first, b bits each, two's complement.
+// When a tile starts on a multiple of 32 codes its stream words are the
+// reference's words verbatim.
+//
+// An apply is two launches:
+//  pass 1  kRight items (workgroups of independent waves) and, per row tile,
+//          the kDense and kFused items, whose row sums go to the workspace;
+//  pass 2  per row tile, the kLeft items plus the pass-1 row sums, written
+//          to y through the permutation.
 #pragma once
 
 #include <cstdint>
@@ -20,48 +42,35 @@
 namespace qhamd {
 
 constexpr int kTile = 64;          // rows per row tile, columns per column tile
-constexpr int kRowWaves = 16;      // waves per row-tile workgroup
-constexpr int kStageWords = 1024;  // LDS words a wave stages at a time
-constexpr int kPassComponents = 16;  // components per right-factor pass (4 lanes each)
-constexpr int kUnitWaves = 4;        // right-factor units (one per wave) per workgroup
+constexpr int kWaves = 16;         // waves per workgroup (wave lists are cut for 16)
+constexpr int kItemQuads = 256;    // 16-byte groups of data per item (4 KiB)
+constexpr int kItemComponents = 64;  // components per item
+constexpr int kDenseColumns = 8;     // columns per kDense item (64 x 8 FP64 = 4 KiB)
 
-// Low-rank block with rank > 0.  Right-factor partial sums of column tile s,
-// component j live in workspace row zbase + s * rank + j.
-struct BlockDev {
-    std::uint32_t rank, comp_base;
-    std::uint32_t zbase, nseg;
+enum ItemKind : std::uint8_t { kDense = 0, kFused = 1, kRight = 2, kLeft = 3 };
+
+// One item (32 bytes).  Fields not listed for a kind are zero.
+struct ItemDev {
+    std::uint32_t quad;     // first 16-byte group of the data in `words`
+    std::uint16_t quads;    // 16-byte groups of data (<= kItemQuads)
+    std::uint8_t kind;      // ItemKind
+    std::uint8_t count;     // kDense: columns; otherwise components (<= 64)
+    std::uint32_t comp0;    // first component (index into widths / cscale)
+    std::uint32_t col;      // kDense / kFused / kRight: first column (clustered index)
+    std::uint32_t cols;     // kFused / kRight: columns of the block in this tile
+    std::uint32_t zrow;     // kRight: workspace row of comp0; kLeft: same, column tile 0
+    std::uint32_t zstride;  // kLeft: workspace rows between column tiles (the block rank)
+    std::uint32_t nseg;     // kLeft: column tiles of the block
 };
+static_assert(sizeof(ItemDev) == 32, "ItemDev is 32 bytes");
 
-// Right-factor unit: block `block` restricted to columns [col0, col0+cols).
-struct UnitDev {
-    std::uint32_t block, col0, cols, zrow;
-    std::uint32_t words, pad;  // words of the chunk (all components, unpadded)
-    std::uint64_t chunk;       // first word, multiple of 4
-};
-
-// Row-tile entry: kDenseBit | dense index, or low-rank block index.
-struct EntryDev {
-    std::uint32_t what;
-    std::uint32_t row;    // dense: first tile row relative to the block
-    std::uint32_t words;  // low-rank: words of the chunk (all components); dense: first column
-    std::uint32_t cols;   // dense: columns of this entry (<= kDenseCols)
-    std::uint64_t chunk;  // low-rank: first word of the block's chunk for this tile
-};
-constexpr int kDenseCols = 16;  // columns of a diagonal block per row-tile entry
-constexpr std::uint32_t kDenseBit = 0x80000000u;
-
-// Row tile: rows [row0, row0+rows); entries [split[w], split[w+1]) are summed
-// by wave w and the waves are combined in order 0..kRowWaves-1.
+// Row tile: rows [row0, row0+rows).  Pass-1 items [split1[w], split1[w+1])
+// and pass-2 items [split2[w], split2[w+1]) belong to wave w (of kWaves);
+// a launch with kWaves/2 waves gives wave w the runs of 2w and 2w+1.
 struct TileDev {
     std::uint32_t row0, rows;
-    std::uint32_t split[kRowWaves + 1];
-    std::uint32_t pad;
-};
-
-// Diagonal block: m x n FP64 stored column-major at values[off].
-struct DenseDev {
-    std::uint32_t row0, col0, m, n;
-    std::uint64_t off;
+    std::uint32_t split1[kWaves + 1];
+    std::uint32_t split2[kWaves + 1];
 };
 
 // One store's device arrays plus one call's operands: everything a work item
@@ -72,27 +81,26 @@ struct MemberView {
     const std::int32_t* perm;
     const std::uint32_t* words;
     const std::int32_t* widths;
-    const double* lscale;
-    const double* rscale;
-    const double* dense;
-    const BlockDev* blocks;
-    const DenseDev* denses;
-    const UnitDev* units;
+    const double* cscale;        // lscale * rscale per component
+    const ItemDev* items;
     const TileDev* tiles;
-    const EntryDev* entries;
+    const std::uint32_t* rsplit;  // right waves: items [rsplit[w], rsplit[w+1])
     const double* x;
     double* y;
-    double* zp;
+    double* zp;     // workspace: z_rows x k partial sums, then order x k row sums
     std::int64_t ldx, ldy;
     std::int32_t k;
-    std::uint32_t units_count, tiles_count;
-    std::uint32_t pad;
+    std::uint32_t tiles_count;  // row tiles
+    std::uint32_t rgroups;      // pass-1 workgroups of right waves
+    std::uint32_t rwaves;       // right waves (rsplit has rwaves + 1 entries)
+    std::uint64_t z_rows;       // first workspace row of the row sums
 };
 
-// Host record of one chunk (test hook and statistics).
+// Host record of where one factor side of a block's codes lives: components
+// [j0, j0+jn) of rows/columns [first, first+count) start at word `word`.
 struct ChunkRef {
-    std::uint32_t first, count;  // rows/columns of the block it holds
-    std::uint64_t chunk;
+    std::uint32_t first, count, j0, jn;
+    std::uint64_t word;
 };
 
 struct DeviceStore {
@@ -102,18 +110,14 @@ struct DeviceStore {
     std::int32_t* perm = nullptr;
     std::uint32_t* words = nullptr;
     std::int32_t* widths = nullptr;
-    double* lscale = nullptr;
-    double* rscale = nullptr;
-    double* dense = nullptr;
-    BlockDev* blocks = nullptr;
-    DenseDev* denses = nullptr;
-    UnitDev* units = nullptr;
+    double* cscale = nullptr;
+    ItemDev* items = nullptr;
     TileDev* tiles = nullptr;
-    EntryDev* entries = nullptr;
+    std::uint32_t* rsplit = nullptr;
     // sizes
-    std::uint64_t n_words = 0, n_comp = 0, n_dense_values = 0;
-    std::uint32_t n_lowrank = 0, n_dense = 0, n_units = 0, n_tiles = 0, n_entries = 0;
-    std::uint64_t z_rows = 0;  // workspace = z_rows * k doubles
+    std::uint64_t n_words = 0, n_comp = 0;
+    std::uint32_t n_tiles = 0, n_items = 0, n_rwaves = 0, n_rgroups = 0;
+    std::uint64_t z_rows = 0;  // workspace = (z_rows + order) * k doubles
     // host mirrors needed by entry points
     std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;  // per low-rank block
@@ -134,25 +138,25 @@ struct DeviceStore {
 struct Plan {
     std::uint64_t order = 0;
     std::vector<std::int32_t> perm;
-    std::vector<std::uint32_t> words;  // chunks + 8 words of padding
+    std::vector<std::uint32_t> words;  // item data, 16-byte aligned runs, + padding
     std::vector<std::int32_t> widths;
-    std::vector<double> lscale, rscale, dense;
-    std::vector<BlockDev> blocks;
-    std::vector<DenseDev> denses;
-    std::vector<UnitDev> units;
+    std::vector<double> cscale;
+    std::vector<ItemDev> items;
     std::vector<TileDev> tiles;
-    std::vector<EntryDev> entries;
+    std::vector<std::uint32_t> rsplit;
+    std::uint32_t rgroups = 0;
     std::vector<std::int64_t> block_to_lowrank;
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;
+    std::vector<std::uint32_t> comp_base, ranks;
     std::uint64_t z_rows = 0, stored_words = 0;
     qh_store_stats stats{};
 };
 
 // Throw std::invalid_argument / std::runtime_error like the host store.
 Plan build_plan(const qh_store_desc& desc);
-// Runs the two passes' index arithmetic on the host against the plan's
-// arrays, with every array access bounds-checked (std::out_of_range), and
-// returns y = H x in FP64.  A check of the layout without a device.
+// Runs both passes' index arithmetic on the host against the plan's arrays,
+// with every array access bounds-checked (std::out_of_range), and returns
+// y = H x in FP64.  A check of the layout without a device.
 void emulate_plan(const Plan& plan, const double* x, std::int64_t k, double* y);
 DeviceStore* create_device_store(const qh_store_desc& desc, int device);
 
@@ -162,9 +166,9 @@ struct BatchClass {
     int kb = 1;
     std::uint64_t member_count = 0;
     std::int64_t max_k = 1;
-    std::uint32_t total_units = 0, total_tiles = 0;
-    MemberView* members = nullptr;        // device table; x, y, zp hold offsets
-    std::uint32_t* first_unit = nullptr;  // device, member_count + 1 prefix counts
+    std::uint32_t total_groups1 = 0, total_tiles = 0;
+    MemberView* members = nullptr;          // device table; x, y, zp hold offsets
+    std::uint32_t* first_group1 = nullptr;  // device, member_count + 1 prefix counts
     std::uint32_t* first_tile = nullptr;
 };
 

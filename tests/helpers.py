@@ -87,3 +87,34 @@ def oracle_of(native) -> O.OracleStore:
 
 def rel_err(a, b) -> float:
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def wide_strips(tiles: int = 68, rank: int = 3, seed: int = 7):
+    """Row strips of 64: a dense 64x64 diagonal block between two low-rank
+    blocks spanning the rest of the strip.  The first and last strips hold
+    blocks wider than 64 column tiles (their partial sums come in several
+    runs); every strip also has a high-rank 64x64 block split over items."""
+    rng = np.random.default_rng(seed)
+    n = 64 * tiles
+    native = q.NativeHMatrix(n, list(rng.permutation(n)))
+    menu = [4, 8, 12, 16, 24, 32]
+
+    def low_rank(r0, r1, c0, c1, r):
+        u = np.linalg.qr(rng.normal(size=(r1 - r0, r)))[0]
+        v = np.linalg.qr(rng.normal(size=(c1 - c0, r)))[0]
+        sigma = np.sort(rng.uniform(0.5, 2.0, size=r))[::-1]
+        bits = list(rng.choice(menu, size=r))
+        native.add_low_rank(q.Rectangle(r0, r1, c0, c1), u, sigma, v, bits, len(menu), 0.0)
+
+    for i in range(tiles):
+        r0, r1 = 64 * i, 64 * i + 64
+        native.add_dense(q.Rectangle(r0, r1, r0, r1), rng.normal(size=(64, 64)))
+        if i > 0:
+            if i == 1:
+                low_rank(r0, r1, 0, 64, 40)  # 64x64 with many 32-bit-capable components
+            else:
+                low_rank(r0, r1, 0, r0, rank)
+        if i < tiles - 1:
+            low_rank(r0, r1, r1, n, rank)
+    native.finalize()
+    return native

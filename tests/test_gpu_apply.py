@@ -35,6 +35,7 @@ def _stores():
     out["line10"] = H.line_kernel(10, 2, 0.25)[0].native
     out["ragged_square"] = H.ragged_square().native
     out["user_partition"] = H.user_partition()[0].native
+    out["wide_strips"] = H.wide_strips()
     return out
 
 
@@ -53,7 +54,7 @@ def line_inverse(torch):
 
 
 NAMES = ["two_by_two"] + [f"orthogonal_b{b}" for b in (2, 4, 8, 12, 16, 24, 32)] + [
-    "line12", "line10", "ragged_square", "user_partition"]
+    "line12", "line10", "ragged_square", "user_partition", "wide_strips"]
 
 
 def _check_codes(native):

@@ -22,6 +22,7 @@ def _cases():
     yield "ragged_square", H.ragged_square(n=700, leaf=24).native
     yield "user_partition", H.user_partition()[0].native
     yield "line_wide_leaves", H.line_kernel(300, 100, 1e-6, widths=(4, 8, 12, 16, 24, 32))[0].native
+    yield "wide_strips", H.wide_strips()
     for p in sorted(GOLDEN.glob("*.qhmat")):
         yield p.stem, qhmat.HMatrix.load(p).native
 
