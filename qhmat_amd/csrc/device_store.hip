@@ -5,15 +5,14 @@
 // deterministic (fixed summation order, one writer per output element, no
 // floating-point atomics):
 //
-//  pass 1  workgroups [0, rgroups) are independent waves streaming kRight
-//          items: partial sums z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c]
-//          into the workspace.  Workgroups from rgroups on own one row tile
-//          each: their waves sum the tile's diagonal (kDense) and fused
-//          (kFused: z and U z inside the item) contributions, and the
-//          waves' rows are added in wave order into the workspace.
+//  pass 1  one workgroup per row tile: its waves sum the tile's diagonal
+//          (kDense) and fused (kFused: z and U z inside the item)
+//          contributions, added in a fixed tree into the workspace, and
+//          stream a share of the kRight items: partial sums
+//          z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c] into the workspace.
 //  pass 2  one workgroup per row tile: the kLeft items (U z with z summed
-//          over the block's column tiles), added to the pass-1 rows in wave
-//          order and written once to y[perm[row]].
+//          over the block's column tiles), added to the pass-1 rows and
+//          written once to y[perm[row]].
 //
 // Every wave walks a contiguous list of items with the next item's data
 // (<= 4 KiB, four 16-byte loads per lane), component widths, scales and
@@ -22,12 +21,14 @@
 // next are requested with the next item's data, so no load waits on another
 // inside the loop.
 //
-// Decoding: a code is v_alignbit_b32 of two staged words + v_bfe_i32.  FP64
-// mode computes in FP64 throughout.  FP32 split mode converts codes of at
-// most 24 bits and x to FP32 (exact for the codes), splits wider codes into
-// a signed high half and an unsigned 16-bit low half accumulated apart, sums
-// each item's contribution to a row in FP32 and everything across items, the
-// scales and the diagonal in FP64.
+// Decoding: a code is v_alignbit_b32 of two staged words and a sign-extending
+// shift pair, branch-free for every width.  FP64 mode computes in FP64
+// throughout.  FP32 split mode converts codes of at most 24 bits and x to
+// FP32 (exact for the codes); where wider codes occur (per group of
+// components) it splits every code into a high half and a sign-extended low
+// 16-bit half, both exact in FP32, accumulated apart and joined in FP64.  An
+// item's contribution to a row is summed in FP32; everything across items,
+// the scales and the diagonal stay in FP64.
 //
 // Reference: src/qhmat/cuda.cu:64-164 (one thread per output entry scanning
 // every block, store uploaded per call) and core.cpp:360-390 (FP64 loop nest).
@@ -106,14 +107,19 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive prefix sum over the 64 lanes.
-__device__ __forceinline__ std::uint32_t wave_scan(std::uint32_t v, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const std::uint32_t t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
-    return v;
+// Inclusive prefix sum over the 64 lanes, all in DPP: shifted adds within
+// each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and
+// row 2's into row 3 (row_bcast15), then rows 0-1's total into rows 2-3
+// (row_bcast31).  A lane without a source adds the zero passed as `old`.
+__device__ __forceinline__ std::uint32_t wave_scan(std::uint32_t v, int) {
+    int x = int(v);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+    return std::uint32_t(x);
 }
 
 // 16-byte group as a native clang vector (an array of HIP's union-based
@@ -122,9 +128,12 @@ typedef std::uint32_t quad_t __attribute__((ext_vector_type(4)));
 
 template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
 
-// LDS of one wave: the current item's data (plus one readable quad past the
-// end) and its inputs (x rows, or the workspace partial sums of kLeft).
-template <int KB> constexpr int wave_lds_bytes() { return (kItemQuads + 1) * 16 + kTile * KB * 8; }
+// LDS of one wave: the current item's data plus slack that lanes decoding
+// past the end of a stream may read (64 codes of 32 bits and the word after),
+// then the item's inputs (x rows of a diagonal item, or the workspace
+// partial sums of kLeft).
+constexpr int kStageBytes = kItemQuads * 16 + 272;
+template <int KB> constexpr int wave_lds_bytes() { return kStageBytes + kTile * KB * 8; }
 
 // Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
 // items of the group).
@@ -198,141 +207,353 @@ __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::i
     }
 }
 
-// Sum over one column tile of a right-factor stream: lanes (g, q) hold
-// component g of the round, columns q, q+4, ...; the quad's four partial
-// sums are added (lanes 4g..4g+3 all end with the total).
+// Code `idx` of a stream whose first word is at `base` (LDS): the word pair
+// holding the code (v_alignbit_b32), then a shift pair that sign-extends the
+// b-bit field (also for b = 32).  No branches: lanes past the end of a
+// stream read the stage's slack words.
+__device__ __forceinline__ int decode_lane(const std::uint32_t* base, std::uint32_t bits,
+                                           std::uint32_t idx) {
+    const std::uint32_t pos = __umul24(idx, bits);
+    const std::uint32_t* w = base + (pos >> 5);
+    const std::uint32_t raw = __builtin_amdgcn_alignbit(w[1], w[0], pos);
+    const std::uint32_t sh = 32u - bits;
+    return static_cast<int>(raw << sh) >> sh;
+}
+
+// Per-component uniform fields packed in one lane value: stream offset in
+// words (< 2^16) and width.  Lanes past the item's components hold a dummy
+// (offset = end of the streams, 32 bits) whose input is zero.
+__device__ __forceinline__ std::uint32_t pack_meta(bool comp, std::uint32_t off, std::uint32_t bits) {
+    return off | ((comp ? bits : 32u) << 16);
+}
+
+// Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
+// lane-half swaps (v_permlane32_swap / v_permlane16_swap).
+template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, 0xF, 0xF, false));
+    const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, 0xF, 0xF, false));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+// (a, b) -> (a, b) with lanes 32-63 of a exchanged for lanes 0-31 of b.
+__device__ __forceinline__ void swap_halves32(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_halves32(double& a, double& b) {
+    const unsigned long long x = static_cast<unsigned long long>(__double_as_longlong(a));
+    const unsigned long long y = static_cast<unsigned long long>(__double_as_longlong(b));
+    const auto lo = __builtin_amdgcn_permlane32_swap(unsigned(x), unsigned(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(unsigned(x >> 32), unsigned(y >> 32), false, false);
+    a = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi[0]) << 32) | lo[0]));
+    b = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi[1]) << 32) | lo[1]));
+}
+// (a, b) -> (a, b) with the odd 16-lane rows of a exchanged for the even rows of b.
+__device__ __forceinline__ void swap_rows16(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_rows16(double& a, double& b) {
+    const unsigned long long x = static_cast<unsigned long long>(__double_as_longlong(a));
+    const unsigned long long y = static_cast<unsigned long long>(__double_as_longlong(b));
+    const auto lo = __builtin_amdgcn_permlane16_swap(unsigned(x), unsigned(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(unsigned(x >> 32), unsigned(y >> 32), false, false);
+    a = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi[0]) << 32) | lo[0]));
+    b = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi[1]) << 32) | lo[1]));
+}
+
+constexpr int kDppRowMirror = 0x140;      // lane i <- lane 15 - i within 16
+constexpr int kDppRowHalfMirror = 0x141;  // lane i <- lane 7 - i within 8
+constexpr int kDppQuadXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppQuadXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+
+// Sums N (16, 8 or 4) per-lane values over the 64 lanes.  A butterfly that
+// halves the values at each level, every lane keeping one half and taking
+// its partner's share of it: lane halves (partner lane +-32,
+// v_permlane32_swap), 16-lane rows (+-16, v_permlane16_swap), the mirrored
+// lane within 16 and within 8 (DPP row_mirror / row_half_mirror); once one
+// value is left the remaining levels just add, ending with the quad (DPP).
+// Lane L ends with the total of value block_slot<N>(L); every value's total is
+// formed by the same tree.
+template <int N, class T>
+__device__ __forceinline__ T reduce_block(const T (&v)[N], int lane) {
+    static_assert(N == 16 || N == 8 || N == 4, "block of 4, 8 or 16 values");
+    T w[N / 2];
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        T a = v[2 * i], b = v[2 * i + 1];
+        swap_halves32(a, b);  // lanes 0-31: value 2i; lanes 32-63: value 2i+1
+        w[i] = a + b;
+    }
+    T u[N / 4];
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+        T a = w[2 * i], b = w[2 * i + 1];
+        swap_rows16(a, b);  // even rows: w[2i]; odd rows: w[2i+1]
+        u[i] = a + b;
+    }
+    const bool b3 = lane & 8, b2 = lane & 4;
+    T s;
+    if constexpr (N == 4) {
+        s = u[0] + dpp_move<kDppRowMirror>(u[0]);
+        s += dpp_move<kDppRowHalfMirror>(s);
+    } else if constexpr (N == 8) {
+        const T t = (b3 ? u[1] : u[0]) + dpp_move<kDppRowMirror>(b3 ? u[0] : u[1]);
+        s = t + dpp_move<kDppRowHalfMirror>(t);
+    } else {
+        T t[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            t[i] = (b3 ? u[2 * i + 1] : u[2 * i]) + dpp_move<kDppRowMirror>(b3 ? u[2 * i] : u[2 * i + 1]);
+        s = (b2 ? t[1] : t[0]) + dpp_move<kDppRowHalfMirror>(b2 ? t[0] : t[1]);
+    }
+    s += dpp_move<kDppQuadXor2>(s);
+    s += dpp_move<kDppQuadXor1>(s);
+    return s;
+}
+
+// Value whose total lane L holds after reduce_block<N>, and the first lane
+// holding value v.
+template <int N>
+__host__ __device__ __forceinline__ int block_slot(int lane) {
+    const int b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
+    return N == 16 ? 8 * b2 + 4 * b3 + 2 * b4 + b5 : N == 8 ? 4 * b3 + 2 * b4 + b5 : 2 * b4 + b5;
+}
+template <int N>
+__host__ __device__ __forceinline__ int block_lane(int v) {
+    const int s5 = v & 1, s4 = (v >> 1) & 1, s3 = (v >> 2) & 1, s2 = (v >> 3) & 1;
+    return N == 16 ? 32 * s5 + 16 * s4 + 8 * s3 + 4 * s2 : N == 8 ? 32 * s5 + 16 * s4 + 8 * s3
+                                                                  : 32 * s5 + 16 * s4;
+}
+// Lanes that write the totals of a block: one per value.
+template <int N>
+__host__ __device__ __forceinline__ bool block_writer(int lane) {
+    return (lane & (N == 16 ? 3 : N == 8 ? 7 : 15)) == 0;
+}
+
+// A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
+// exact in FP32, and a code of small magnitude stays in l (no cancellation
+// between the halves).
+__device__ __forceinline__ void split_code(int code, float& h, float& l) {
+    // (code - l) / 2^16 without leaving 32 bits: the arithmetic high half,
+    // plus one when the low half is negative
+    h = float((code >> 16) + ((code >> 15) & 1));
+    l = float(static_cast<int>(__builtin_amdgcn_sbfe(code, 0u, 16u)));
+}
+
+// Row contributions of four left-factor components (lanes are rows); zs[u]
+// is component u's scaled input, zero for padding components.  In FP32 mode
+// the four take the split path when a real one of them is wider than 24
+// bits (groups of four start at multiples of four, whatever the
+// right-hand-side count).
 template <int KB, bool FP32>
-__device__ __forceinline__ void right_sums(const std::uint32_t* __restrict__ stage,
-                                           const real_t<FP32>* __restrict__ xr,
-                                           std::uint32_t cols, std::uint32_t bits,
-                                           std::uint32_t off, bool mine, int q, double (&tot)[KB]) {
-    using Real = real_t<FP32>;
-    const std::uint32_t* sw = stage + off;
-    if (!FP32 || bits <= 24) {
-        Real part[KB];
+__device__ __forceinline__ void left_four(const std::uint32_t* stage, const std::uint32_t (&meta)[4],
+                                          bool wide, int lane, const real_t<FP32> (&zs)[4][KB],
+                                          double (&acc)[KB], float (&blk)[KB]) {
+    int code[4];
 #pragma unroll
-        for (int p = 0; p < KB; ++p) part[p] = Real(0);
-        if (mine) {
-            for (std::uint32_t c = q; c < cols; c += 4) {
-                const Real code = Real(decode_code(sw, c, bits));
+    for (int u = 0; u < 4; ++u) code[u] = decode_lane(stage + (meta[u] & 0xFFFFu), meta[u] >> 16, lane);
+    if (!FP32) {
 #pragma unroll
-                for (int p = 0; p < KB; ++p) part[p] = fma(code, xr[c * KB + p], part[p]);
-            }
+        for (int u = 0; u < 4; ++u) {
+            const double cd = double(code[u]);
+#pragma unroll
+            for (int p = 0; p < KB; ++p) acc[p] = fma(cd, double(zs[u][p]), acc[p]);
         }
+    } else if (!wide) {
 #pragma unroll
-        for (int p = 0; p < KB; ++p) {
-            Real v = part[p];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            tot[p] = double(v);
+        for (int u = 0; u < 4; ++u) {
+            const float cf = float(code[u]);
+#pragma unroll
+            for (int p = 0; p < KB; ++p) blk[p] = fmaf(cf, float(zs[u][p]), blk[p]);
         }
     } else {
-        // 25..32-bit codes: signed high half and unsigned low 16 bits, both
-        // exact in FP32, summed apart and joined in FP64.
-        float hi[KB], lo[KB];
 #pragma unroll
-        for (int p = 0; p < KB; ++p) hi[p] = lo[p] = 0.0f;
-        if (mine) {
-            for (std::uint32_t c = q; c < cols; c += 4) {
-                const int code = decode_code(sw, c, bits);
-                const float h = float(code >> 16), l = float(code & 0xFFFF);
+        for (int u = 0; u < 4; ++u) {
+            float h, l;
+            split_code(code[u], h, l);
 #pragma unroll
-                for (int p = 0; p < KB; ++p) {
-                    hi[p] = fmaf(h, float(xr[c * KB + p]), hi[p]);
-                    lo[p] = fmaf(l, float(xr[c * KB + p]), lo[p]);
-                }
+            for (int p = 0; p < KB; ++p) {
+                blk[p] = fmaf(h, float(zs[u][p]) * 65536.0f, blk[p]);
+                blk[p] = fmaf(l, float(zs[u][p]), blk[p]);
             }
-        }
-#pragma unroll
-        for (int p = 0; p < KB; ++p) {
-            float h = hi[p], l = lo[p];
-            h += __shfl_xor(h, 1, 64);
-            h += __shfl_xor(h, 2, 64);
-            l += __shfl_xor(l, 1, 64);
-            l += __shfl_xor(l, 2, 64);
-            tot[p] = double(h) * 65536.0 + double(l);
         }
     }
 }
 
-// Row contribution of one left-factor component: lanes are rows; zs is the
-// component's scaled input (lane-uniform).
-template <int KB, bool FP32>
-__device__ __forceinline__ void left_component(const std::uint32_t* __restrict__ stage,
-                                               std::uint32_t off, std::uint32_t bits, bool active,
-                                               int lane, const real_t<FP32> (&zs)[KB],
-                                               double (&acc)[KB], float (&blk)[KB]) {
-    const int code = active ? decode_code(stage + off, lane, bits) : 0;
-    if (!FP32) {
-        const double cd = double(code);
+// Left-factor rows of components [j0, jend) of an item, four at a time:
+// meta held one per lane (lanes = components), zs_of(j, p) the scaled input
+// of component j for right-hand side p (lane-uniform).
+template <int KB, bool FP32, class ZsOf>
+__device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint32_t lmeta_l,
+                                          std::uint32_t j0, std::uint32_t jend, int lane,
+                                          ZsOf zs_of, double (&acc)[KB], float (&blk)[KB]) {
+    using Real = real_t<FP32>;
+    for (std::uint32_t j = j0; j < jend; j += 4) {
+        std::uint32_t meta[4];
+        Real zs[4][KB];
+        bool wide = false;
 #pragma unroll
-        for (int p = 0; p < KB; ++p) acc[p] = fma(cd, double(zs[p]), acc[p]);
-    } else if (bits <= 24) {
-        const float cf = float(code);
+        for (int u = 0; u < 4; ++u) {
+            const bool real = j + u < jend;
+            meta[u] = __builtin_amdgcn_readlane(lmeta_l, min(j + u, 63u));
+            wide = wide || (real && (meta[u] >> 16) > 24u);
 #pragma unroll
-        for (int p = 0; p < KB; ++p) blk[p] = fmaf(cf, float(zs[p]), blk[p]);
-    } else {
-        const float h = float(code >> 16), l = float(code & 0xFFFF);
+            for (int p = 0; p < KB; ++p) zs[u][p] = real ? zs_of(j + u, p) : Real(0);
+        }
+        left_four<KB, FP32>(stage, meta, wide, lane, zs, acc, blk);
+    }
+}
+
+// Right-factor sums of a block of N components [j0, j0 + N) of an item
+// (fewer real ones at the end: the rest decode the dummy stream and are
+// never used): lanes are the item's columns, x rows in xl (zero past the
+// last column).  tot[p] in lane L is the sum for component
+// j0 + block_slot<N>(L) and right-hand side p.  In FP32 mode, when a real
+// component of the block is wider than 24 bits, every code of the block is
+// split (split_code) and the halves are summed apart and joined in FP64.
+// The blocks depend only on the item's component count, so a component's
+// rounding is the same for every right-hand-side count.
+template <int N, int KB, bool FP32>
+__device__ __forceinline__ void right_block(const std::uint32_t* stage, std::uint32_t rmeta_l,
+                                            std::uint32_t j0, std::uint32_t count, int lane,
+                                            const real_t<FP32> (&xl)[KB], double (&tot)[KB]) {
+    using Real = real_t<FP32>;
+    int code[N];
+    bool wide = false;
+#pragma unroll
+    for (int g = 0; g < N; ++g) {
+        const std::uint32_t meta = __builtin_amdgcn_readlane(rmeta_l, min(j0 + g, 63u));
+        wide = wide || (j0 + g < count && (meta >> 16) > 24u);
+        code[g] = decode_lane(stage + (meta & 0xFFFFu), meta >> 16, lane);
+    }
+    if (!FP32 || !wide) {
 #pragma unroll
         for (int p = 0; p < KB; ++p) {
-            blk[p] = fmaf(h, float(zs[p]) * 65536.0f, blk[p]);
-            blk[p] = fmaf(l, float(zs[p]), blk[p]);
+            Real v[N];
+#pragma unroll
+            for (int g = 0; g < N; ++g) v[g] = Real(code[g]) * xl[p];
+            tot[p] = double(reduce_block<N>(v, lane));
         }
+        return;
     }
+    float h[N], l[N];
+#pragma unroll
+    for (int g = 0; g < N; ++g) split_code(code[g], h[g], l[g]);
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+        float vh[N], vl[N];
+#pragma unroll
+        for (int g = 0; g < N; ++g) {
+            vh[g] = h[g] * float(xl[p]);
+            vl[g] = l[g] * float(xl[p]);
+        }
+        tot[p] = double(reduce_block<N>(vh, lane)) * 65536.0 + double(reduce_block<N>(vl, lane));
+    }
+}
+
+// One block of N components of a kRight or kFused item: right-factor sums,
+// then (kRight) their workspace rows or (kFused) the scaled inputs and the
+// left streams of the same components.
+template <int N, int KB, bool FP32>
+__device__ __forceinline__ void right_then_left(const MemberView& M, const Item& f,
+                                                const std::uint32_t* stage, std::uint32_t rmeta_l,
+                                                std::uint32_t lmeta_l, double scale_l,
+                                                std::uint32_t j0, int lane, int p0, int kk,
+                                                const real_t<FP32> (&xl)[KB], double (&acc)[KB],
+                                                float (&blk)[KB]) {
+    using Real = real_t<FP32>;
+    double tot[KB];
+    right_block<N, KB, FP32>(stage, rmeta_l, j0, f.count, lane, xl, tot);
+    const std::uint32_t jl = j0 + std::uint32_t(block_slot<N>(lane));
+    if (f.kind == kRight) {
+        if (block_writer<N>(lane) && jl < f.count) {
+            double* out = M.zp + (std::uint64_t(f.zrow) + jl) * std::uint64_t(M.k) + p0;
+#pragma unroll
+            for (int p = 0; p < KB; ++p)
+                if (p < kk) out[p] = tot[p];
+        }
+        return;
+    }
+    const double sc = __shfl(scale_l, int(min(jl, 63u)), 64);
+    Real zs_lane[KB];
+#pragma unroll
+    for (int p = 0; p < KB; ++p) zs_lane[p] = Real(sc * tot[p]);
+    left_rows<KB, FP32>(stage, lmeta_l, j0, min(j0 + N, f.count), lane,
+                        [&](std::uint32_t j, int p) {
+                            return lane_value(zs_lane[p], block_lane<N>(int(j - j0)));
+                        },
+                        acc, blk);
 }
 
 // One wave's items [b, e).  Row contributions (kDense, kFused, kLeft) are
 // added to acc (lanes = the R rows of the tile); kRight items write their
 // partial sums to the workspace.
 template <int KB, bool FP32>
-__device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b, std::uint32_t e,
-                                          int p0, int kk, std::uint32_t R,
-                                          std::uint8_t* __restrict__ wave_lds, double (&acc)[KB]) {
+__device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1, std::uint32_t e1,
+                                          std::uint32_t b2, std::uint32_t e2, int p0, int kk,
+                                          std::uint32_t R, std::uint8_t* __restrict__ wave_lds,
+                                          double (&acc)[KB]) {
     using Real = real_t<FP32>;
     const int lane = threadIdx.x & 63;
-    const bool active = lane < int(R);
-    std::uint32_t* stage = reinterpret_cast<std::uint32_t*>(wave_lds);
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds);
     quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds);
-    double* xs = reinterpret_cast<double*>(wave_lds + (kItemQuads + 1) * 16);
-    Real* xr = reinterpret_cast<Real*>(xs);
+    double* xs = reinterpret_cast<double*>(wave_lds + kStageBytes);
     const std::int64_t k = M.k;
 
-    for (std::uint32_t g0 = b; g0 < e; g0 += 64) {
-        const std::uint32_t n = min(64u, e - g0);
+    // the list is [b1, e1) followed by [b2, e2)
+    const std::uint32_t n1 = e1 - b1, total = n1 + (e2 - b2);
+    for (std::uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const std::uint32_t n = min(64u, total - g0);
         quad_t r0 = {0u, 0u, 0u, 0u}, r1 = {0u, 0u, 0u, 0u};
         if (std::uint32_t(lane) < n) {
-            const quad_t* rp = reinterpret_cast<const quad_t*>(M.items + g0 + lane);
+            const std::uint32_t i = g0 + lane;
+            const quad_t* rp = reinterpret_cast<const quad_t*>(M.items + (i < n1 ? b1 + i : b2 + (i - n1)));
             r0 = rp[0];
             r1 = rp[1];
         }
+        Item f = item_at(r0, r1, 0);
+        Item nxt = n > 1 ? item_at(r0, r1, 1) : f;
         Fetch<KB> F;
-        fetch<KB>(M, item_at(r0, r1, 0), perm_row(M, item_at(r0, r1, 0), lane), lane, p0, kk, F);
-        std::int32_t prow = n > 1 ? perm_row(M, item_at(r0, r1, 1), lane) : 0;
+        fetch<KB>(M, f, perm_row(M, f, lane), lane, p0, kk, F);
+        std::int32_t prow = n > 1 ? perm_row(M, nxt, lane) : 0;
 
         for (std::uint32_t s = 0; s < n; ++s) {
-            const Item f = item_at(r0, r1, s);
             // Hand the fetched registers to LDS and keep the per-component ones.
 #pragma unroll
             for (int t = 0; t < kItemQuads / 64; ++t)
                 if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
+            const bool comp = std::uint32_t(lane) < f.count;
+            const std::uint32_t bits_l = F.bits;
+            const double scale_l = F.scale;
+            double in[KB];
+#pragma unroll
+            for (int p = 0; p < KB; ++p) in[p] = F.in[p];
             if (f.kind == kDense || f.kind == kLeft) {
                 if (std::uint32_t(lane) < max(x_columns(f), z_pairs(f))) {
 #pragma unroll
-                    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+                    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = in[p];
                 }
-            } else if (lane < int(f.cols)) {
-#pragma unroll
-                for (int p = 0; p < KB; ++p) xr[lane * KB + p] = Real(F.in[p]);
             }
-            const std::uint32_t bits_l = F.bits;
-            const double scale_l = F.scale;
             // Request the next item, and the permutation rows of the one after.
+            Item after = nxt;
             if (s + 1 < n) {
-                fetch<KB>(M, item_at(r0, r1, s + 1), prow, lane, p0, kk, F);
-                if (s + 2 < n) prow = perm_row(M, item_at(r0, r1, s + 2), lane);
+                fetch<KB>(M, nxt, prow, lane, p0, kk, F);
+                if (s + 2 < n) {
+                    after = item_at(r0, r1, s + 2);
+                    prow = perm_row(M, after, lane);
+                }
             }
             wave_lds_sync();
 
+            float blk[KB];
+#pragma unroll
+            for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
             if (f.kind == kDense) {
                 // lanes = rows; column c of the item at stage[2 (c R + row)]
                 const double* dv = reinterpret_cast<const double*>(stage);
@@ -351,7 +572,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b, 
 #pragma unroll
                 for (int p = 0; p < KB; ++p) z[p] = 0.0;
                 const std::uint32_t nseg = f.nseg;
-                if (std::uint32_t(lane) < f.count && (lane + 1) * nseg <= 64u) {
+                if (comp && (lane + 1) * nseg <= 64u) {
                     for (std::uint32_t sg = 0; sg < nseg; ++sg) {
 #pragma unroll
                         for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
@@ -360,73 +581,46 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b, 
                 Real zs_l[KB];
 #pragma unroll
                 for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
-                const std::uint32_t nw = std::uint32_t(lane) < f.count ? stream_words(R, bits_l) : 0u;
-                const std::uint32_t off_l = wave_scan(nw, lane) - nw;
-                float blk[KB];
-#pragma unroll
-                for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
-                for (std::uint32_t j = 0; j < f.count; ++j) {
-                    Real zs[KB];
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) zs[p] = lane_value(zs_l[p], j);
-                    left_component<KB, FP32>(stage, __builtin_amdgcn_readlane(off_l, j),
-                                             __builtin_amdgcn_readlane(bits_l, j), active, lane,
-                                             zs, acc, blk);
-                }
-                if (FP32) {
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
-                }
+                const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
+                const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
+                left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
+                                    [&](std::uint32_t j, int p) { return lane_value(zs_l[p], j); },
+                                    acc, blk);
             } else {
-                // kFused / kRight: right-factor sums in rounds of 16 components
-                const std::uint32_t nwr =
-                    std::uint32_t(lane) < f.count ? stream_words(f.cols, bits_l) : 0u;
+                // kFused / kRight: lanes = columns for the right factor
+                Real xl[KB];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) xl[p] = lane < int(f.cols) ? Real(in[p]) : Real(0);
+                const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
                 const std::uint32_t incl_r = wave_scan(nwr, lane);
-                const std::uint32_t roff_l = incl_r - nwr;
+                const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
                 const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
-                const std::uint32_t nwl =
-                    std::uint32_t(lane) < f.count ? stream_words(R, bits_l) : 0u;
-                const std::uint32_t loff_l = rtotal + wave_scan(nwl, lane) - nwl;
-                const int g = lane >> 2, q = lane & 3;
-                float blk[KB];
+                const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
+                const std::uint32_t lmeta_l =
+                    pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
+                // blocks of 16 components, then one of 8 or 4 for the rest
+                std::uint32_t j0 = 0;
+                for (; j0 + 16 <= f.count; j0 += 16)
+                    right_then_left<16, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
+                                                  p0, kk, xl, acc, blk);
+                const std::uint32_t rest = f.count - j0;
+                if (rest > 8)
+                    right_then_left<16, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
+                                                  p0, kk, xl, acc, blk);
+                else if (rest > 4)
+                    right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
+                                                 p0, kk, xl, acc, blk);
+                else if (rest > 0)
+                    right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
+                                                 p0, kk, xl, acc, blk);
+            }
+            if (FP32 && f.kind != kDense) {
 #pragma unroll
-                for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
-                for (std::uint32_t j0 = 0; j0 < f.count; j0 += 16) {
-                    const std::uint32_t jn = min(16u, f.count - j0);
-                    const int src = int(j0) + g;
-                    const std::uint32_t bits_g = __shfl(bits_l, src, 64);
-                    const std::uint32_t off_g = __shfl(roff_l, src, 64);
-                    const bool mine = std::uint32_t(g) < jn;
-                    double tot[KB];
-                    right_sums<KB, FP32>(stage, xr, f.cols, bits_g, off_g, mine, q, tot);
-                    if (f.kind == kRight) {
-                        if (mine && q == 0) {
-                            double* out = M.zp + (std::uint64_t(f.zrow) + j0 + g) * std::uint64_t(k) + p0;
-#pragma unroll
-                            for (int p = 0; p < KB; ++p)
-                                if (p < kk) out[p] = tot[p];
-                        }
-                        continue;
-                    }
-                    const double scale_g = __shfl(scale_l, src, 64);
-                    Real zs_g[KB];
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) zs_g[p] = Real(scale_g * tot[p]);
-                    for (std::uint32_t j = 0; j < jn; ++j) {
-                        Real zs[KB];
-#pragma unroll
-                        for (int p = 0; p < KB; ++p) zs[p] = lane_value(zs_g[p], 4 * j);
-                        left_component<KB, FP32>(stage, __builtin_amdgcn_readlane(loff_l, j0 + j),
-                                                 __builtin_amdgcn_readlane(bits_l, j0 + j), active,
-                                                 lane, zs, acc, blk);
-                    }
-                }
-                if (FP32 && f.kind == kFused) {
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
-                }
+                for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
             }
             wave_lds_sync();  // the next item overwrites the stage
+            f = nxt;
+            nxt = after;
         }
     }
 }
@@ -438,8 +632,9 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b, 
 // lower levels of the tree in registers, the rest in wave 0; the result is
 // the same bits for every W (so for every right-hand-side count).
 template <int KB, bool FP32, int W>
-__device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32_t* split, int wave,
-                                          int p0, int kk, std::uint32_t R, std::uint8_t* wave_lds,
+__device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32_t* rsplit,
+                                          const std::uint32_t* split, int wave, int p0, int kk,
+                                          std::uint32_t R, std::uint8_t* wave_lds,
                                           double (&total)[KB]) {
     constexpr int step = kWaves / W;
     double pair[KB];
@@ -448,8 +643,9 @@ __device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32
         double acc[KB];
 #pragma unroll
         for (int p = 0; p < KB; ++p) acc[p] = 0.0;
-        run_items<KB, FP32>(M, split[wave * step + i], split[wave * step + i + 1], p0, kk, R,
-                            wave_lds, acc);
+        const int r = wave * step + i;
+        run_items<KB, FP32>(M, rsplit ? rsplit[r] : 0u, rsplit ? rsplit[r + 1] : 0u, split[r],
+                            split[r + 1], p0, kk, R, wave_lds, acc);
 #pragma unroll
         for (int p = 0; p < KB; ++p) {
             if (i == 0) total[p] = acc[p];
@@ -497,25 +693,16 @@ __device__ __forceinline__ void combine_rows(const double (&total)[KB], const do
 }
 
 template <int KB, bool FP32, int W>
-__device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t group, int p0,
-                                            std::uint8_t* lds) {
+__device__ __forceinline__ void pass1_tile(const MemberView& M, std::uint32_t t, int p0,
+                                           std::uint8_t* lds) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int kk = min(KB, M.k - p0);
-    std::uint8_t* mine = lds + wave * wave_lds_bytes<KB>();
-    double total[KB];
-    if (group < M.rgroups) {  // independent right-factor waves
-        const std::uint32_t w = group * W + wave;
-        if (w < M.rwaves) {
-#pragma unroll
-            for (int p = 0; p < KB; ++p) total[p] = 0.0;
-            run_items<KB, FP32>(M, M.rsplit[w], M.rsplit[w + 1], p0, kk, 0, mine, total);
-        }
-        return;
-    }
-    const TileDev* __restrict__ tile = M.tiles + (group - M.rgroups);
+    const TileDev* __restrict__ tile = M.tiles + t;
     const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
-    wave_rows<KB, FP32, W>(M, tile->split1, wave, p0, kk, R, mine, total);
+    double total[KB];
+    wave_rows<KB, FP32, W>(M, tile->rsplit, tile->split1, wave, p0, kk, R,
+                           lds + wave * wave_lds_bytes<KB>(), total);
     double zero[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zero[p] = 0.0;
@@ -544,8 +731,8 @@ __device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t,
     for (int p = 0; p < KB; ++p) base[p] = (wave == 0 && active && p < kk) ? rows[p] : 0.0;
     const std::int64_t yrow = active ? std::int64_t(M.perm[tile->row0 + lane]) * M.ldy + p0 : 0;
     double total[KB];
-    wave_rows<KB, FP32, W>(M, tile->split2, wave, p0, kk, R, lds + wave * wave_lds_bytes<KB>(),
-                           total);
+    wave_rows<KB, FP32, W>(M, nullptr, tile->split2, wave, p0, kk, R,
+                           lds + wave * wave_lds_bytes<KB>(), total);
     double* y = M.y;
     combine_rows<KB, W>(total, base, lds, wave, lane, [&](int pc, double a0, double a1) {
         if (active) {
@@ -558,7 +745,7 @@ __device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t,
 template <int KB, bool FP32, int W>
 __global__ __launch_bounds__(64 * W) void apply_pass1(MemberView M) {
     __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
-    pass1_group<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
+    pass1_tile<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
 template <int KB, bool FP32, int W>
@@ -601,7 +788,7 @@ __global__ __launch_bounds__(64 * W) void batch_pass1(const MemberView* __restri
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
-    pass1_group<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
+    pass1_tile<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
 }
 
 template <int KB, bool FP32, int W>
@@ -637,7 +824,7 @@ __global__ void decode_side(const std::uint32_t* __restrict__ words,
 // Host side
 
 MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
-                       std::int64_t ldy, double* zp, int waves) {
+                       std::int64_t ldy, double* zp) {
     MemberView M{};
     M.perm = s.perm;
     M.words = s.words;
@@ -645,7 +832,6 @@ MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, 
     M.cscale = s.cscale;
     M.items = s.items;
     M.tiles = s.tiles;
-    M.rsplit = s.rsplit;
     M.x = x;
     M.y = y;
     M.zp = zp;
@@ -653,8 +839,6 @@ MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, 
     M.ldy = ldy;
     M.k = k;
     M.tiles_count = s.n_tiles;
-    M.rwaves = s.n_rwaves;
-    M.rgroups = (s.n_rwaves + waves - 1) / waves;
     M.z_rows = s.z_rows;
     return M;
 }
@@ -669,18 +853,18 @@ hipEvent_t* profile_slot(const DeviceStore& s) {
 // Waves per workgroup: fewer waves with more registers each as the
 // right-hand sides per pass grow (the wave lists are cut for 16 waves; a
 // wave of a smaller workgroup takes 16/W consecutive lists).
-constexpr int waves_for(int kb) { return kb >= 16 ? 4 : kb >= 8 ? 8 : 16; }
+constexpr int waves_for(int kb) { return kb >= 16 ? 4 : kb >= 2 ? 8 : 16; }
 
 template <int KB, bool FP32>
 void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
                   std::int64_t ldy, double* zp, hipStream_t stream) {
     constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((k + KB - 1) / KB);
-    const MemberView M = member_view(s, x, ldx, k, y, ldy, zp, W);
+    const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
-    hipLaunchKernelGGL((apply_pass1<KB, FP32, W>), dim3(M.rgroups + s.n_tiles, passes),
-                       dim3(64 * W), 0, stream, M);
+    hipLaunchKernelGGL((apply_pass1<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
+                       stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
     hipLaunchKernelGGL((apply_pass2<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
@@ -695,8 +879,8 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
     constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
-    hipLaunchKernelGGL((batch_pass1<KB, FP32, W>), dim3(c.total_groups1, passes), dim3(64 * W),
-                       0, stream, c.members, c.first_group1, count, x, y, zp);
+    hipLaunchKernelGGL((batch_pass1<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
+                       stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 1 launch");
     hipLaunchKernelGGL((batch_pass2<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
                        stream, c.members, c.first_tile, count, x, y, zp);
@@ -1037,7 +1221,7 @@ Plan build_plan(const qh_store_desc& d) {
     std::vector<ItemDev>& items = plan.items;
     items = right_items;
     plan.tiles.resize(row_tiles.size());
-    std::uint64_t tile_cost = 0;
+    std::vector<std::uint64_t> run_cost;  // pass-1 cost of every (tile, run)
     for (std::size_t t = 0; t < row_tiles.size(); ++t) {
         TileDev& T = plan.tiles[t];
         T.row0 = std::uint32_t(row_tiles[t].first);
@@ -1046,27 +1230,38 @@ Plan build_plan(const qh_store_desc& d) {
             const auto& list = pass == 0 ? pass1[t] : pass2[t];
             std::vector<std::uint64_t> cost;
             for (const auto& it : list) cost.push_back(item_cost(it));
+            std::uint32_t* split = pass == 0 ? T.split1 : T.split2;
+            split_runs(cost, kWaves, std::uint32_t(items.size()), split);
             if (pass == 0)
-                for (auto c : cost) tile_cost += c;
-            split_runs(cost, kWaves, std::uint32_t(items.size()), pass == 0 ? T.split1 : T.split2);
+                for (int w = 0; w < kWaves; ++w) {
+                    std::uint64_t c = 0;
+                    for (std::uint32_t i = split[w]; i < split[w + 1]; ++i)
+                        c += cost[i - split[0]];
+                    run_cost.push_back(c);
+                }
             items.insert(items.end(), list.begin(), list.end());
         }
     }
     require(items.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
-    // Right waves carry about what a pass-1 tile wave carries.
-    std::vector<std::uint64_t> rcost;
-    std::uint64_t rtotal = 0;
-    for (const auto& it : right_items) {
-        rcost.push_back(item_cost(it));
-        rtotal += rcost.back();
-    }
-    if (!right_items.empty()) {
-        const std::uint64_t target = std::max<std::uint64_t>(
-            8 * kItemCost, tile_cost / std::max<std::uint64_t>(1, kWaves * row_tiles.size()));
-        const std::uint64_t waves =
-            std::min<std::uint64_t>(right_items.size(), (rtotal + target - 1) / target);
-        plan.rsplit.resize(waves + 1);
-        split_runs(rcost, std::uint32_t(waves), 0, plan.rsplit.data());
+    // The right-factor items fill the pass-1 runs up to an even total: run r
+    // takes items (in order) while its cost stays under the mean.
+    std::uint64_t all = 0;
+    for (auto c : run_cost) all += c;
+    for (const auto& it : right_items) all += item_cost(it);
+    const std::uint64_t mean = all / std::max<std::size_t>(1, run_cost.size());
+    std::uint32_t next = 0;
+    for (std::size_t r = 0; r < run_cost.size(); ++r) {
+        TileDev& T = plan.tiles[r / kWaves];
+        const int w = int(r % kWaves);
+        T.rsplit[w] = next;
+        std::uint64_t c = run_cost[r];
+        const bool last = r + 1 == run_cost.size();
+        while (next < right_items.size() &&
+               (last || c + item_cost(right_items[next]) / 2 <= mean)) {
+            c += item_cost(right_items[next]);
+            ++next;
+        }
+        T.rsplit[w + 1] = next;
     }
     plan.z_rows = z_rows;
 
@@ -1087,7 +1282,10 @@ Plan build_plan(const qh_store_desc& d) {
     st.dense_flops_per_rhs = dn_flops;
     st.tiles = plan.tiles.size();
     st.items = items.size();
-    st.right_waves = plan.rsplit.empty() ? 0 : plan.rsplit.size() - 1;
+    std::uint64_t right_runs = 0;
+    for (const auto& T : plan.tiles)
+        for (int w = 0; w < kWaves; ++w) right_runs += T.rsplit[w + 1] > T.rsplit[w] ? 1 : 0;
+    st.right_waves = right_runs;
     st.pass_code_words[0] = pass_words[0];
     st.pass_code_words[1] = pass_words[1];
     st.fused_components = fused_components;
@@ -1112,7 +1310,6 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
         s->cscale = upload(plan.cscale, bytes);
         s->items = upload(plan.items, bytes);
         s->tiles = upload(plan.tiles, bytes);
-        s->rsplit = upload(plan.rsplit, bytes);
         s->device_bytes = bytes;
     } catch (...) {
         destroy_device_store(s);
@@ -1122,7 +1319,6 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     s->n_comp = plan.widths.size();
     s->n_tiles = std::uint32_t(plan.tiles.size());
     s->n_items = std::uint32_t(plan.items.size());
-    s->n_rwaves = plan.rsplit.empty() ? 0u : std::uint32_t(plan.rsplit.size() - 1);
     s->z_rows = plan.z_rows;
     s->block_to_lowrank = std::move(plan.block_to_lowrank);
     s->left_chunks = std::move(plan.left_chunks);
@@ -1202,9 +1398,11 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
         }
     };
     std::vector<double> z;
-    // Pass 1, right waves.
-    for (std::size_t w = 0; w + 1 < pl.rsplit.size(); ++w)
-        for (std::uint32_t i = at(pl.rsplit, w); i < at(pl.rsplit, w + 1); ++i) {
+    // Pass 1, the right-factor runs of every tile.
+    for (const TileDev& T : pl.tiles)
+        for (std::uint32_t i = T.rsplit[0]; i < T.rsplit[kWaves]; ++i) {
+            for (int w = 0; w < kWaves; ++w)
+                check(T.rsplit[w] <= T.rsplit[w + 1], "right runs out of order");
             const ItemDev& it = at(pl.items, i);
             check_item(it);
             check(it.kind == kRight, "right wave holds another item kind");
@@ -1280,8 +1478,7 @@ void destroy_device_store(DeviceStore* s) {
     for (void* e : s->profile.events) (void)hipEventDestroy(static_cast<hipEvent_t>(e));
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
                     static_cast<void*>(s->widths), static_cast<void*>(s->cscale),
-                    static_cast<void*>(s->items), static_cast<void*>(s->tiles),
-                    static_cast<void*>(s->rsplit)})
+                    static_cast<void*>(s->items), static_cast<void*>(s->tiles)})
         if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete s;
@@ -1415,9 +1612,8 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
         for (int kb : {1, 2, 4, 8, 16}) {
             BatchClass c;
             c.kb = kb;
-            const int W = waves_for(kb);
             std::vector<MemberView> views;
-            std::vector<std::uint32_t> fg{0}, ft{0};
+            std::vector<std::uint32_t> ft{0};
             for (std::uint64_t i = 0; i < count; ++i) {
                 if (rhs_class(b->k[i]) != kb) continue;
                 const DeviceStore& s = *b->stores[i];
@@ -1425,22 +1621,17 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
                 views.push_back(member_view(
                     s, reinterpret_cast<const double*>(std::uintptr_t(b->offset[i])), b->k[i],
                     int(b->k[i]), reinterpret_cast<double*>(std::uintptr_t(b->offset[i])),
-                    b->k[i], reinterpret_cast<double*>(std::uintptr_t(b->zoffset[i])), W));
-                const std::uint64_t groups = views.back().rgroups + std::uint64_t(s.n_tiles);
-                require(std::uint64_t(fg.back()) + groups < (std::uint64_t{1} << 31) &&
-                            std::uint64_t(ft.back()) + s.n_tiles < (std::uint64_t{1} << 31),
+                    b->k[i], reinterpret_cast<double*>(std::uintptr_t(b->zoffset[i]))));
+                require(std::uint64_t(ft.back()) + s.n_tiles < (std::uint64_t{1} << 31),
                         "batch too large for one launch");
-                fg.push_back(fg.back() + std::uint32_t(groups));
                 ft.push_back(ft.back() + s.n_tiles);
                 c.max_k = std::max(c.max_k, b->k[i]);
             }
             if (views.empty()) continue;
             c.member_count = views.size();
-            c.total_groups1 = fg.back();
             c.total_tiles = ft.back();
             std::uint64_t bytes = 0;
             c.members = upload(views, bytes);
-            c.first_group1 = upload(fg, bytes);
             c.first_tile = upload(ft, bytes);
             b->classes.push_back(c);
         }
@@ -1457,8 +1648,7 @@ void destroy_batch(Batch* b) {
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(b->device);
     for (auto& c : b->classes)
-        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_group1),
-                        static_cast<void*>(c.first_tile)})
+        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_tile)})
             if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete b;

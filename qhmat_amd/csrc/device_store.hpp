@@ -27,11 +27,11 @@
 // When a tile starts on a multiple of 32 codes its stream words are the
 // reference's words verbatim.
 //
-// An apply is two launches:
-//  pass 1  kRight items (workgroups of independent waves) and, per row tile,
-//          the kDense and kFused items, whose row sums go to the workspace;
-//  pass 2  per row tile, the kLeft items plus the pass-1 row sums, written
-//          to y through the permutation.
+// An apply is two launches of one workgroup per row tile:
+//  pass 1  the tile's kDense and kFused items, whose row sums go to the
+//          workspace, and a share of the kRight items;
+//  pass 2  the tile's kLeft items plus the pass-1 row sums, written to y
+//          through the permutation.
 #pragma once
 
 #include <cstdint>
@@ -64,11 +64,15 @@ struct ItemDev {
 };
 static_assert(sizeof(ItemDev) == 32, "ItemDev is 32 bytes");
 
-// Row tile: rows [row0, row0+rows).  Pass-1 items [split1[w], split1[w+1])
-// and pass-2 items [split2[w], split2[w+1]) belong to wave w (of kWaves);
-// a launch with kWaves/2 waves gives wave w the runs of 2w and 2w+1.
+// Row tile: rows [row0, row0+rows).  Run w (of kWaves) of the tile's pass-1
+// workgroup streams the kRight items [rsplit[w], rsplit[w+1]) (a share of
+// all right factors, independent of the tile) and the tile's items
+// [split1[w], split1[w+1]); run w of pass 2 the items [split2[w],
+// split2[w+1]).  A launch with W < kWaves waves gives each wave kWaves/W
+// consecutive runs.
 struct TileDev {
     std::uint32_t row0, rows;
+    std::uint32_t rsplit[kWaves + 1];
     std::uint32_t split1[kWaves + 1];
     std::uint32_t split2[kWaves + 1];
 };
@@ -84,15 +88,12 @@ struct MemberView {
     const double* cscale;        // lscale * rscale per component
     const ItemDev* items;
     const TileDev* tiles;
-    const std::uint32_t* rsplit;  // right waves: items [rsplit[w], rsplit[w+1])
     const double* x;
     double* y;
     double* zp;     // workspace: z_rows x k partial sums, then order x k row sums
     std::int64_t ldx, ldy;
     std::int32_t k;
     std::uint32_t tiles_count;  // row tiles
-    std::uint32_t rgroups;      // pass-1 workgroups of right waves
-    std::uint32_t rwaves;       // right waves (rsplit has rwaves + 1 entries)
     std::uint64_t z_rows;       // first workspace row of the row sums
 };
 
@@ -113,10 +114,9 @@ struct DeviceStore {
     double* cscale = nullptr;
     ItemDev* items = nullptr;
     TileDev* tiles = nullptr;
-    std::uint32_t* rsplit = nullptr;
     // sizes
     std::uint64_t n_words = 0, n_comp = 0;
-    std::uint32_t n_tiles = 0, n_items = 0, n_rwaves = 0, n_rgroups = 0;
+    std::uint32_t n_tiles = 0, n_items = 0;
     std::uint64_t z_rows = 0;  // workspace = (z_rows + order) * k doubles
     // host mirrors needed by entry points
     std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
@@ -143,8 +143,6 @@ struct Plan {
     std::vector<double> cscale;
     std::vector<ItemDev> items;
     std::vector<TileDev> tiles;
-    std::vector<std::uint32_t> rsplit;
-    std::uint32_t rgroups = 0;
     std::vector<std::int64_t> block_to_lowrank;
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;
     std::vector<std::uint32_t> comp_base, ranks;
@@ -166,10 +164,9 @@ struct BatchClass {
     int kb = 1;
     std::uint64_t member_count = 0;
     std::int64_t max_k = 1;
-    std::uint32_t total_groups1 = 0, total_tiles = 0;
-    MemberView* members = nullptr;          // device table; x, y, zp hold offsets
-    std::uint32_t* first_group1 = nullptr;  // device, member_count + 1 prefix counts
-    std::uint32_t* first_tile = nullptr;
+    std::uint32_t total_tiles = 0;
+    MemberView* members = nullptr;        // device table; x, y, zp hold offsets
+    std::uint32_t* first_tile = nullptr;  // device, member_count + 1 prefix counts
 };
 
 // A fixed set of resident stores applied together.  Member i reads its
