@@ -123,6 +123,98 @@ def cpu_baseline(h, x1, budget_s: float):
     return {"kind": kind, "seconds": statistics.median(times), "runs": len(times)}, y
 
 
+def batch_stores(count: int, seed: int = 4):
+    """Stores of uneven order, kernel, dimension and tolerance (a reduced
+    version of the 64-store batch: n log-uniform in [1024, 4096]) with
+    right-hand-side counts uniform in 1..16."""
+    import numpy as np
+
+    from qhmat_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    kernels = [W.inverse_shifted, W.exponential, W.log_kernel]
+    natives, ks, shapes = [], [], []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(4096)))))
+        dim = 1 + i % 3
+        tol = (1e-4, 1e-6)[i % 2]
+        pts = W.uniform_points(n, dim, seed=100 + i)
+        h = W.build_kernel_hmatrix(pts, kernels[i % 3], tolerance=tol, leaf_size=64)
+        natives.append(h.native)
+        ks.append(int(rng.integers(1, 17)))
+        shapes.append((n, dim, kernels[i % 3].__name__, tol))
+    return natives, ks, shapes
+
+
+def measure_batch(torch, natives, ks, repeats: int = 50) -> dict:
+    """Tunes the stores, then times the batch three ways: every store applied
+    on its own, the batched launch pair per class, and the same batch
+    replayed as one captured HIP graph (FP32 split)."""
+    import ctypes as C
+
+    from qhmat_amd import _native as N
+    from qhmat_amd import batch as B
+    from qhmat_amd import device as D
+    from qhmat_amd import tune as T
+
+    tuned = T.tune_batch(natives, ks, precision="fp32split", repeats=5)
+    batch = B.DeviceBatch(natives, ks)
+    xs = [torch.randn(n.order, k, dtype=torch.float64, device="cuda") for n, k in zip(natives, ks)]
+    x = batch.pack(xs)
+    y = batch.empty()
+    ws = batch.workspace()
+    graph = batch.graph(x, y, precision="fp32split", workspace=ws)
+    lib = N.lib()
+    stream = torch.cuda.current_stream()
+    ys = [torch.empty_like(t) for t in xs]
+    wss = [torch.empty(max(D.workspace_bytes(n, k), 8), dtype=torch.uint8, device="cuda")
+           for n, k in zip(natives, ks)]
+
+    def plain():
+        for n, k, xi, yi, wi in zip(natives, ks, xs, ys, wss):
+            N.check(lib.qh_apply(n.device_store(0), C.c_void_p(xi.data_ptr()), k, k,
+                                 C.c_void_p(yi.data_ptr()), k, N.PRECISION_FP32SPLIT,
+                                 C.c_void_p(stream.cuda_stream), C.c_void_p(wi.data_ptr()),
+                                 wi.numel()))
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(repeats):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / repeats
+
+    t_plain = timed(plain)
+    t_batch = timed(lambda: batch.apply(x, y, precision="fp32split", workspace=ws, stream=stream))
+    t_graph = timed(lambda: graph.launch(stream))
+    matvecs = sum(ks)
+    return {"stores": len(natives), "matvecs_per_launch": matvecs,
+            "plain_ms": round(t_plain, 4), "batched_ms": round(t_batch, 4),
+            "graph_ms": round(t_graph, 4),
+            "graph_matvecs_per_s": round(matvecs / (t_graph * 1e-3), 1),
+            "plain_matvecs_per_s": round(matvecs / (t_plain * 1e-3), 1),
+            "tuned_rhs_block": {f"store{i}_k{r.k}": r.best for i, r in enumerate(tuned)}}
+
+
+def measure_serving(torch, natives: dict, rate: float, seconds: float) -> dict:
+    """Open-loop Poisson arrivals over resident stores with continuous
+    batching (qhmat_amd.serve); k in {1, 4, 16}, sessions of 50 requests."""
+    from qhmat_amd import serve as S
+
+    server = S.Server(device=0, hbm_budget=64 << 30, precision="fp32split", slots=3)
+    orders = {}
+    for name, native in natives.items():
+        server.admit(name, native)
+        orders[name] = native.order
+    with server:
+        out = S.poisson_load(server, list(natives), orders, rate=rate, duration=seconds)
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out.items()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +228,11 @@ def main() -> int:
                     help="first copy 256 MiB device-to-device once (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--rhs", type=int, nargs="+", default=[1, 32])
+    ap.add_argument("--batch-stores", type=int, default=12,
+                    help="stores in the batched-graph measurement (0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of the served-load measurement (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=4000.0)
     args = ap.parse_args()
     if 1 not in args.rhs:
         args.rhs = [1] + args.rhs  # the headline value is quoted at k=1
@@ -186,6 +283,23 @@ def main() -> int:
             results[(k, prec_name)] = {"ms": ms, "passes_ms": passes, "y": out}
             log(f"k={k} {prec_name}: {ms * 1e3:.1f} us/apply "
                 f"(passes {passes[0] * 1e3:.1f} + {passes[1] * 1e3:.1f} us)")
+
+    batched = None
+    if args.batch_stores > 0:
+        log(f"batch: building {args.batch_stores} stores")
+        bn, bk, _ = batch_stores(args.batch_stores)
+        batched = measure_batch(torch, bn, bk)
+        log(f"batch: plain {batched['plain_ms']:.3f} ms, graph {batched['graph_ms']:.3f} ms "
+            f"for {batched['matvecs_per_launch']} matvecs")
+    serving = None
+    if args.serve_seconds > 0:
+        log("serving: building 3 smaller stores")
+        served = {"square_exp_16384": native}
+        for m in (2048, 4096, 8192):
+            served[f"square_exp_{m}"] = W.square_exp(m).build().native
+        serving = measure_serving(torch, served, args.serve_rate, args.serve_seconds)
+        log(f"serving: p50 {serving.get('latency_p50_ms')} ms, p99 {serving.get('latency_p99_ms')} ms "
+            f"at {serving['throughput_rps']:.0f} req/s")
 
     base = None
     y_cpu = None
@@ -252,6 +366,12 @@ def main() -> int:
     for k in args.rhs:
         for prec in ("fp32split", "fp64"):
             line[f"{prec}_k{k}"] = summary(k, prec)
+    if batched is not None:
+        line["batched_graph"] = batched
+    if serving is not None:
+        line["served"] = serving
+        line["served_p50_ms"] = serving.get("latency_p50_ms")
+        line["served_p99_ms"] = serving.get("latency_p99_ms")
     if base is not None:
         line["cpu_baseline"] = {
             "value": round(b1["total"] / base["seconds"] / 1e9, 4), "unit": "GB/s",

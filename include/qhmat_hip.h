@@ -112,6 +112,13 @@ qh_status qh_store_create(const qh_store_desc *desc, int device, qh_store **out)
 qh_status qh_store_destroy(qh_store *store);
 qh_status qh_store_get_stats(const qh_store *store, qh_store_stats *out);
 
+/* Launch-shape choice (no counterpart in the reference): applies of `k`
+ * right-hand sides run in passes of `kb` in {1, 2, 4, 8, 16} (0 restores the
+ * default, k rounded up to a power of two, at most 16).  Results are bitwise
+ * the same for every kb; the autotuner (qhmat_amd/tune.py) times the choices
+ * and keeps the fastest.  Batches created afterwards group members by it. */
+qh_status qh_store_set_rhs_block(qh_store *store, int64_t k, int kb);
+
 /* Workspace bytes qh_apply needs for `k` right-hand sides. */
 qh_status qh_apply_workspace_bytes(const qh_store *store, int64_t k, uint64_t *bytes);
 

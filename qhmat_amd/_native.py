@@ -108,6 +108,7 @@ _sig("qh_apply_workspace_bytes", _p, _i64, C.POINTER(_u64))
 _sig("qh_apply", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p, _p, _u64)
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
 _sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)
+_sig("qh_store_set_rhs_block", _p, _i64, C.c_int)
 _sig("qh_profile_begin", _p, _u64)
 _sig("qh_profile_end", _p, C.POINTER(Profile))
 _sig("qh_host_create", _u64, _p, C.POINTER(_p))
@@ -454,6 +455,12 @@ class NativeHMatrix:
                 check(_lib.qh_store_create(C.byref(desc), int(device), C.byref(handle)))
                 store = self._dev[device] = handle
             return store
+
+    def set_rhs_block(self, k: int, kb: int, device: int = 0) -> None:
+        """Applies of k right-hand sides on `device` run in passes of kb
+        (1, 2, 4, 8 or 16; 0 restores the default).  Bitwise the same result
+        for every kb; see qhmat_amd.tune for choosing it."""
+        check(_lib.qh_store_set_rhs_block(self.device_store(device), int(k), int(kb)))
 
     def upload_copy(self, device: int = 0):
         """A further, independent device copy (the caller frees it with

@@ -229,6 +229,13 @@ qh_status qh_plan_emulate(const qh_store_desc* desc, const double* x, int64_t k,
     });
 }
 
+qh_status qh_store_set_rhs_block(qh_store* store, int64_t k, int kb) {
+    return guarded([&] {
+        need(store, "store");
+        qhamd::set_rhs_block(*store->dev, k, kb);
+    });
+}
+
 qh_status qh_profile_begin(qh_store* store, uint64_t max_applies) {
     return guarded([&] {
         need(store, "store");

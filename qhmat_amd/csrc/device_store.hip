@@ -177,34 +177,61 @@ template <int KB> struct Fetch {
     double in[KB];       // x rows (lanes = columns) or partial sums (lanes = pairs)
 };
 
+// Loads go through buffer resources (V#) sized to the bytes a request may
+// touch: lanes outside an item (or past its columns / components) use an
+// out-of-range offset and read zero, so no load is predicated and every
+// load of an item is issued, whatever its shape (the compiler then counts
+// the loads in flight exactly, which lets two items be in flight).
+constexpr int kBufferFlags = 0x00020000;      // dword 3 of a raw 32-bit buffer resource
+constexpr std::uint32_t kOutOfRange = 0x7FFFFFF0u;  // >= every num_records used here
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void* base, std::uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), kBufferFlags);
+}
+__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, std::uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return __longlong_as_double(static_cast<long long>(
+        (static_cast<unsigned long long>(v[1]) << 32) | static_cast<unsigned long long>(v[0])));
+}
+
 __device__ __forceinline__ std::int32_t perm_row(const MemberView& M, const Item& f, int lane) {
-    return lane < int(x_columns(f)) ? M.perm[f.col + lane] : 0;
+    const std::uint32_t n = x_columns(f);
+    const __amdgpu_buffer_rsrc_t r = buffer(M.perm + f.col, n * 4u);
+    return std::int32_t(__builtin_amdgcn_raw_buffer_load_b32(r, std::uint32_t(lane) * 4u, 0, 0));
 }
 
 template <int KB>
 __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::int32_t prow,
                                       int lane, int p0, int kk, Fetch<KB>& F) {
-    const quad_t* src = reinterpret_cast<const quad_t*>(M.words) + f.quad;
-    const quad_t zero = {0u, 0u, 0u, 0u};
+    const __amdgpu_buffer_rsrc_t data =
+        buffer(M.words + 4ull * f.quad, f.quads * 16u);
 #pragma unroll
     for (int t = 0; t < kItemQuads / 64; ++t)
-        F.q[t] = std::uint32_t(lane + 64 * t) < f.quads ? src[lane + 64 * t] : zero;
-    const bool comp = f.kind != kDense && lane < int(f.count);
-    F.bits = comp ? std::uint32_t(M.widths[f.comp0 + lane]) : 0u;
-    F.scale = (comp && f.kind != kRight) ? M.cscale[f.comp0 + lane] : 0.0;
-    const std::int64_t k = M.k;
+        F.q[t] = __builtin_amdgcn_raw_buffer_load_b128(data, std::uint32_t(lane + 64 * t) * 16u, 0, 0);
+    const std::uint32_t ncomp = f.kind == kDense ? 0u : f.count;
+    F.bits = __builtin_amdgcn_raw_buffer_load_b32(buffer(M.widths + f.comp0, ncomp * 4u),
+                                                  std::uint32_t(lane) * 4u, 0, 0);
+    const std::uint32_t nscale = (f.kind == kFused || f.kind == kLeft) ? f.count : 0u;
+    F.scale = load_f64(buffer(M.cscale + f.comp0, nscale * 8u), std::uint32_t(lane) * 8u);
+    // x and the workspace: byte offsets from the call's base pointers (the
+    // host keeps both below kOutOfRange bytes)
+    const std::uint32_t k = std::uint32_t(M.k);
+    std::uint32_t row;
+    bool mine;
+    __amdgpu_buffer_rsrc_t r;
     if (f.kind == kLeft) {
         const std::uint32_t s = lane % f.nseg, j = lane / f.nseg;
-        const bool mine = std::uint32_t(lane) < z_pairs(f);
-        const double* zr = M.zp + (std::uint64_t(f.zrow) + std::uint64_t(s) * f.zstride + j) * k + p0;
-#pragma unroll
-        for (int p = 0; p < KB; ++p) F.in[p] = (mine && p < kk) ? zr[p] : 0.0;
+        mine = std::uint32_t(lane) < z_pairs(f);
+        row = (f.zrow + s * f.zstride + j) * k;
+        r = buffer(M.zp, kOutOfRange);
     } else {
-        const bool mine = lane < int(x_columns(f));
-        const double* xr = M.x + std::int64_t(prow) * M.ldx + p0;
-#pragma unroll
-        for (int p = 0; p < KB; ++p) F.in[p] = (mine && p < kk) ? xr[p] : 0.0;
+        mine = lane < int(x_columns(f));
+        row = std::uint32_t(prow) * std::uint32_t(M.ldx);
+        r = buffer(M.x, kOutOfRange);
     }
+#pragma unroll
+    for (int p = 0; p < KB; ++p)
+        F.in[p] = load_f64(r, (mine && p < kk) ? (row + std::uint32_t(p0 + p)) * 8u : kOutOfRange);
 }
 
 // Code `idx` of a stream whose first word is at `base` (LDS): the word pair
@@ -222,9 +249,10 @@ __device__ __forceinline__ int decode_lane(const std::uint32_t* base, std::uint3
 
 // Per-component uniform fields packed in one lane value: stream offset in
 // words (< 2^16) and width.  Lanes past the item's components hold a dummy
-// (offset = end of the streams, 32 bits) whose input is zero.
+// (offset = end of the streams, 1 bit) whose input is zero, so loops over
+// whole groups of components need no tests.
 __device__ __forceinline__ std::uint32_t pack_meta(bool comp, std::uint32_t off, std::uint32_t bits) {
-    return off | ((comp ? bits : 32u) << 16);
+    return off | ((comp ? bits : 1u) << 16);
 }
 
 // Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
@@ -386,9 +414,11 @@ __device__ __forceinline__ void left_four(const std::uint32_t* stage, const std:
     }
 }
 
-// Left-factor rows of components [j0, jend) of an item, four at a time:
+// Left-factor rows of components [j0, jend) of an item, four at a time
+// (j0 a multiple of 4 and jend <= 64, so every lane index stays below 64):
 // meta held one per lane (lanes = components), zs_of(j, p) the scaled input
-// of component j for right-hand side p (lane-uniform).
+// of component j for right-hand side p (lane-uniform), zero for components
+// past jend, whose dummy meta decodes in-bounds 1-bit codes.
 template <int KB, bool FP32, class ZsOf>
 __device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint32_t lmeta_l,
                                           std::uint32_t j0, std::uint32_t jend, int lane,
@@ -400,11 +430,10 @@ __device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint3
         bool wide = false;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool real = j + u < jend;
-            meta[u] = __builtin_amdgcn_readlane(lmeta_l, min(j + u, 63u));
-            wide = wide || (real && (meta[u] >> 16) > 24u);
+            meta[u] = __builtin_amdgcn_readlane(lmeta_l, j + u);
+            wide = wide || (meta[u] >> 16) > 24u;
 #pragma unroll
-            for (int p = 0; p < KB; ++p) zs[u][p] = real ? zs_of(j + u, p) : Real(0);
+            for (int p = 0; p < KB; ++p) zs[u][p] = zs_of(j + u, p);
         }
         left_four<KB, FP32>(stage, meta, wide, lane, zs, acc, blk);
     }
@@ -429,7 +458,7 @@ __device__ __forceinline__ void right_block(const std::uint32_t* stage, std::uin
 #pragma unroll
     for (int g = 0; g < N; ++g) {
         const std::uint32_t meta = __builtin_amdgcn_readlane(rmeta_l, min(j0 + g, 63u));
-        wide = wide || (j0 + g < count && (meta >> 16) > 24u);
+        wide = wide || (meta >> 16) > 24u;
         code[g] = decode_lane(stage + (meta & 0xFFFFu), meta >> 16, lane);
     }
     if (!FP32 || !wide) {
@@ -483,7 +512,7 @@ __device__ __forceinline__ void right_then_left(const MemberView& M, const Item&
     const double sc = __shfl(scale_l, int(min(jl, 63u)), 64);
     Real zs_lane[KB];
 #pragma unroll
-    for (int p = 0; p < KB; ++p) zs_lane[p] = Real(sc * tot[p]);
+    for (int p = 0; p < KB; ++p) zs_lane[p] = jl < f.count ? Real(sc * tot[p]) : Real(0);
     left_rows<KB, FP32>(stage, lmeta_l, j0, min(j0 + N, f.count), lane,
                         [&](std::uint32_t j, int p) {
                             return lane_value(zs_lane[p], block_lane<N>(int(j - j0)));
@@ -491,136 +520,176 @@ __device__ __forceinline__ void right_then_left(const MemberView& M, const Item&
                         acc, blk);
 }
 
-// One wave's items [b, e).  Row contributions (kDense, kFused, kLeft) are
-// added to acc (lanes = the R rows of the tile); kRight items write their
-// partial sums to the workspace.
-template <int KB, bool FP32>
+// The registers of an item that outlive its staging: component widths and
+// scale products (lanes = components) and x rows or partial sums.
+template <int KB> struct Staged {
+    std::uint32_t bits_l;
+    double scale_l;
+    double in[KB];
+};
+
+// Moves a fetched item into the wave's LDS (data, and the x rows of a
+// diagonal item or the partial sums of a kLeft item) and keeps the rest.
+template <int KB>
+__device__ __forceinline__ Staged<KB> stage_item(const Item& f, const Fetch<KB>& F, int lane,
+                                                 std::uint8_t* wave_lds) {
+    quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds);
+    double* xs = reinterpret_cast<double*>(wave_lds + kStageBytes);
+#pragma unroll
+    for (int t = 0; t < kItemQuads / 64; ++t)
+        if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
+    Staged<KB> st;
+    st.bits_l = F.bits;
+    st.scale_l = F.scale;
+#pragma unroll
+    for (int p = 0; p < KB; ++p) st.in[p] = F.in[p];
+    if (f.kind == kDense || f.kind == kLeft) {
+        if (std::uint32_t(lane) < max(x_columns(f), z_pairs(f))) {
+#pragma unroll
+            for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+        }
+    }
+    return st;
+}
+
+// Decodes and applies one staged item.  Pass 1 holds kDense, kFused and
+// kRight items, pass 2 kLeft items only.
+template <int KB, bool FP32, int PASS>
+__device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
+                                             const Staged<KB>& st, int lane, int p0, int kk,
+                                             std::uint32_t R, std::uint8_t* wave_lds,
+                                             double (&acc)[KB]) {
+    using Real = real_t<FP32>;
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds);
+    const double* xs = reinterpret_cast<const double*>(wave_lds + kStageBytes);
+    const bool comp = std::uint32_t(lane) < f.count;
+    const std::uint32_t bits_l = st.bits_l;
+    const double scale_l = st.scale_l;
+    const double (&in)[KB] = st.in;
+    float blk[KB];
+#pragma unroll
+    for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
+    if (PASS == 1 && f.kind == kDense) {
+        // lanes = rows; column c of the item at stage[2 (c R + row)]
+        const double* dv = reinterpret_cast<const double*>(stage);
+        const std::uint32_t cnt = min(f.count, std::uint32_t(kDenseColumns));
+#pragma unroll
+        for (int c = 0; c < kDenseColumns; ++c) {
+            if (std::uint32_t(c) < cnt) {
+                const double v = dv[c * R + lane];
+#pragma unroll
+                for (int p = 0; p < KB; ++p) acc[p] = fma(v, xs[c * KB + p], acc[p]);
+            }
+        }
+    } else if (PASS == 2) {
+        // kLeft. z of each component: its column tiles' partial sums in order
+        double z[KB];
+#pragma unroll
+        for (int p = 0; p < KB; ++p) z[p] = 0.0;
+        const std::uint32_t nseg = f.nseg;
+        if (comp && (lane + 1) * nseg <= 64u) {
+            for (std::uint32_t sg = 0; sg < nseg; ++sg) {
+#pragma unroll
+                for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
+            }
+        }
+        Real zs_l[KB];
+#pragma unroll
+        for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
+        const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
+        const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
+        left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
+                            [&](std::uint32_t j, int p) { return lane_value(zs_l[p], j); },
+                            acc, blk);
+    } else {
+        // kFused / kRight: lanes = columns for the right factor
+        Real xl[KB];
+#pragma unroll
+        for (int p = 0; p < KB; ++p) xl[p] = lane < int(f.cols) ? Real(in[p]) : Real(0);
+        const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
+        const std::uint32_t incl_r = wave_scan(nwr, lane);
+        const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
+        const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
+        const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
+        const std::uint32_t lmeta_l =
+            pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
+        // blocks of 8 components, the last of 8 or 4 (the widest components
+        // come first, so in FP32 mode only the first blocks usually take the
+        // split path)
+        std::uint32_t j0 = 0;
+        for (; j0 + 4 < f.count; j0 += 8)
+            right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0, kk,
+                                         xl, acc, blk);
+        if (j0 < f.count)
+            right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0, kk,
+                                         xl, acc, blk);
+    }
+    if (FP32 && f.kind != kDense) {
+#pragma unroll
+        for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
+    }
+}
+
+// One wave's items: [b1, e1) followed by [b2, e2).  Row contributions
+// (kDense, kFused, kLeft) are added to acc (lanes = the R rows of the tile);
+// kRight items write their partial sums to the workspace.  Two items are in
+// flight while one is decoded: the fetch buffers alternate, and the
+// permutation rows of the next item to fetch are requested one step ahead.
+template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1, std::uint32_t e1,
                                           std::uint32_t b2, std::uint32_t e2, int p0, int kk,
                                           std::uint32_t R, std::uint8_t* __restrict__ wave_lds,
                                           double (&acc)[KB]) {
-    using Real = real_t<FP32>;
     const int lane = threadIdx.x & 63;
-    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds);
-    quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds);
-    double* xs = reinterpret_cast<double*>(wave_lds + kStageBytes);
-    const std::int64_t k = M.k;
-
-    // the list is [b1, e1) followed by [b2, e2)
     const std::uint32_t n1 = e1 - b1, total = n1 + (e2 - b2);
     for (std::uint32_t g0 = 0; g0 < total; g0 += 64) {
         const std::uint32_t n = min(64u, total - g0);
-        quad_t r0 = {0u, 0u, 0u, 0u}, r1 = {0u, 0u, 0u, 0u};
-        if (std::uint32_t(lane) < n) {
-            const std::uint32_t i = g0 + lane;
-            const quad_t* rp = reinterpret_cast<const quad_t*>(M.items + (i < n1 ? b1 + i : b2 + (i - n1)));
-            r0 = rp[0];
-            r1 = rp[1];
-        }
-        Item f = item_at(r0, r1, 0);
-        Item nxt = n > 1 ? item_at(r0, r1, 1) : f;
-        Fetch<KB> F;
-        fetch<KB>(M, f, perm_row(M, f, lane), lane, p0, kk, F);
-        std::int32_t prow = n > 1 ? perm_row(M, nxt, lane) : 0;
+        // item records, lanes = items of this group
+        const std::uint32_t i = g0 + lane;
+        const std::uint32_t idx = i < n1 ? b1 + i : b2 + (i - n1);
+        const __amdgpu_buffer_rsrc_t recs = buffer(M.items, kOutOfRange);
+        const std::uint32_t roff = std::uint32_t(lane) < n ? idx * 32u : kOutOfRange;
+        const quad_t r0 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff, 0, 0);
+        const quad_t r1 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff + 16u, 0, 0);
 
-        for (std::uint32_t s = 0; s < n; ++s) {
-            // Hand the fetched registers to LDS and keep the per-component ones.
-#pragma unroll
-            for (int t = 0; t < kItemQuads / 64; ++t)
-                if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
-            const bool comp = std::uint32_t(lane) < f.count;
-            const std::uint32_t bits_l = F.bits;
-            const double scale_l = F.scale;
-            double in[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) in[p] = F.in[p];
-            if (f.kind == kDense || f.kind == kLeft) {
-                if (std::uint32_t(lane) < max(x_columns(f), z_pairs(f))) {
-#pragma unroll
-                    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = in[p];
-                }
-            }
-            // Request the next item, and the permutation rows of the one after.
-            Item after = nxt;
-            if (s + 1 < n) {
-                fetch<KB>(M, nxt, prow, lane, p0, kk, F);
-                if (s + 2 < n) {
-                    after = item_at(r0, r1, s + 2);
-                    prow = perm_row(M, after, lane);
-                }
-            }
-            wave_lds_sync();
+        // Every step issues the same loads: past the end of the group it
+        // fetches an empty item (all loads out of range, reading zero), so
+        // the loads in flight are counted exactly and never waited for early.
+        const Item none = {};
+        auto slot = [&](std::uint32_t i) { return i < n ? item_at(r0, r1, i) : none; };
+        Item fa = slot(0), fb = slot(1), fc = slot(2);
+        Fetch<KB> FA, FB;
+        fetch<KB>(M, fa, perm_row(M, fa, lane), lane, p0, kk, FA);
+        fetch<KB>(M, fb, perm_row(M, fb, lane), lane, p0, kk, FB);
+        std::int32_t pc = perm_row(M, fc, lane);  // rows of the item fetched next
 
-            float blk[KB];
-#pragma unroll
-            for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
-            if (f.kind == kDense) {
-                // lanes = rows; column c of the item at stage[2 (c R + row)]
-                const double* dv = reinterpret_cast<const double*>(stage);
-                const std::uint32_t cnt = min(f.count, std::uint32_t(kDenseColumns));
-#pragma unroll
-                for (int c = 0; c < kDenseColumns; ++c) {
-                    if (std::uint32_t(c) < cnt) {
-                        const double v = dv[c * R + lane];
-#pragma unroll
-                        for (int p = 0; p < KB; ++p) acc[p] = fma(v, xs[c * KB + p], acc[p]);
-                    }
-                }
-            } else if (f.kind == kLeft) {
-                // z of each component: its column tiles' partial sums in order
-                double z[KB];
-#pragma unroll
-                for (int p = 0; p < KB; ++p) z[p] = 0.0;
-                const std::uint32_t nseg = f.nseg;
-                if (comp && (lane + 1) * nseg <= 64u) {
-                    for (std::uint32_t sg = 0; sg < nseg; ++sg) {
-#pragma unroll
-                        for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
-                    }
-                }
-                Real zs_l[KB];
-#pragma unroll
-                for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
-                const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
-                const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
-                left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
-                                    [&](std::uint32_t j, int p) { return lane_value(zs_l[p], j); },
-                                    acc, blk);
-            } else {
-                // kFused / kRight: lanes = columns for the right factor
-                Real xl[KB];
-#pragma unroll
-                for (int p = 0; p < KB; ++p) xl[p] = lane < int(f.cols) ? Real(in[p]) : Real(0);
-                const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
-                const std::uint32_t incl_r = wave_scan(nwr, lane);
-                const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
-                const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
-                const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
-                const std::uint32_t lmeta_l =
-                    pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
-                // blocks of 16 components, then one of 8 or 4 for the rest
-                std::uint32_t j0 = 0;
-                for (; j0 + 16 <= f.count; j0 += 16)
-                    right_then_left<16, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
-                                                  p0, kk, xl, acc, blk);
-                const std::uint32_t rest = f.count - j0;
-                if (rest > 8)
-                    right_then_left<16, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
-                                                  p0, kk, xl, acc, blk);
-                else if (rest > 4)
-                    right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
-                                                 p0, kk, xl, acc, blk);
-                else if (rest > 0)
-                    right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane,
-                                                 p0, kk, xl, acc, blk);
+        // steady state: buffer A holds item s, buffer B item s+1
+        for (std::uint32_t s = 0; s < n; s += 2) {
+            {
+                const Staged<KB> st = stage_item<KB>(fa, FA, lane, wave_lds);
+                fetch<KB>(M, fc, pc, lane, p0, kk, FA);
+                const Item fd = slot(s + 3);
+                const std::int32_t pd = perm_row(M, fd, lane);
+                wave_lds_sync();
+                compute_item<KB, FP32, PASS>(M, fa, st, lane, p0, kk, R, wave_lds, acc);
+                wave_lds_sync();  // the next item overwrites the stage
+                fa = fc;
+                fc = fd;
+                pc = pd;
             }
-            if (FP32 && f.kind != kDense) {
-#pragma unroll
-                for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
+            if (s + 1 >= n) break;
+            {
+                const Staged<KB> st = stage_item<KB>(fb, FB, lane, wave_lds);
+                fetch<KB>(M, fc, pc, lane, p0, kk, FB);
+                const Item fd = slot(s + 4);
+                const std::int32_t pd = perm_row(M, fd, lane);
+                wave_lds_sync();
+                compute_item<KB, FP32, PASS>(M, fb, st, lane, p0, kk, R, wave_lds, acc);
+                wave_lds_sync();
+                fb = fc;
+                fc = fd;
+                pc = pd;
             }
-            wave_lds_sync();  // the next item overwrites the stage
-            f = nxt;
-            nxt = after;
         }
     }
 }
@@ -644,8 +713,7 @@ __device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32
 #pragma unroll
         for (int p = 0; p < KB; ++p) acc[p] = 0.0;
         const int r = wave * step + i;
-        run_items<KB, FP32>(M, rsplit ? rsplit[r] : 0u, rsplit ? rsplit[r + 1] : 0u, split[r],
-                            split[r + 1], p0, kk, R, wave_lds, acc);
+        run_items<KB, FP32, 2>(M, 0u, 0u, split[r], split[r + 1], p0, kk, R, wave_lds, acc);
 #pragma unroll
         for (int p = 0; p < KB; ++p) {
             if (i == 0) total[p] = acc[p];
@@ -692,21 +760,30 @@ __device__ __forceinline__ void combine_rows(const double (&total)[KB], const do
     }
 }
 
-template <int KB, bool FP32, int W>
-__device__ __forceinline__ void pass1_tile(const MemberView& M, std::uint32_t t, int p0,
-                                           std::uint8_t* lds) {
+// Pass 1: workgroup q (of kPass1Groups) of tile t runs the tile's runs
+// 4q .. 4q+3, one per wave, and writes their tree sum to its own set of
+// rows in the workspace; pass 2 adds the sets as ((g0 + g1) + (g2 + g3)),
+// which completes the same balanced tree over the 16 runs.
+template <int KB, bool FP32>
+__device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t group, int p0,
+                                            std::uint8_t* lds) {
+    constexpr int W = kWaves / kPass1Groups;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int kk = min(KB, M.k - p0);
+    const std::uint32_t t = group / kPass1Groups, q = group % kPass1Groups;
     const TileDev* __restrict__ tile = M.tiles + t;
     const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
+    const int r = int(q) * W + wave;
     double total[KB];
-    wave_rows<KB, FP32, W>(M, tile->rsplit, tile->split1, wave, p0, kk, R,
-                           lds + wave * wave_lds_bytes<KB>(), total);
+#pragma unroll
+    for (int p = 0; p < KB; ++p) total[p] = 0.0;
+    run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
+                        tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(), total);
     double zero[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zero[p] = 0.0;
-    double* rows = M.zp + (M.z_rows + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+    double* rows = M.zp + (M.z_rows + q * M.order + tile->row0 + lane) * std::uint64_t(M.k) + p0;
     combine_rows<KB, W>(total, zero, lds, wave, lane, [&](int pc, double a0, double a1) {
         if (lane < int(R)) {
             if (pc < kk) rows[pc] = a0;
@@ -724,17 +801,31 @@ __device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t,
     const TileDev* __restrict__ tile = M.tiles + t;
     const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
     const bool active = lane < int(R);
-    // pass-1 rows of this tile, requested first
-    double base[KB];
+    // pass-1 rows of this tile, one set per pass-1 workgroup: requested first
+    // for up to two right-hand sides, at the end (fewer live registers) beyond
+    const std::uint64_t stride = std::uint64_t(M.order) * std::uint64_t(M.k);
     const double* rows = M.zp + (M.z_rows + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+    auto pass1_rows = [&](int p) {
+        double g[kPass1Groups];
 #pragma unroll
-    for (int p = 0; p < KB; ++p) base[p] = (wave == 0 && active && p < kk) ? rows[p] : 0.0;
+        for (int q = 0; q < kPass1Groups; ++q)
+            g[q] = (wave == 0 && active && p < kk) ? rows[q * stride + p] : 0.0;
+        return (g[0] + g[1]) + (g[2] + g[3]);
+    };
+    constexpr bool early = KB <= 2;
+    double base[KB];
+#pragma unroll
+    for (int p = 0; p < KB; ++p) base[p] = early ? pass1_rows(p) : 0.0;
     const std::int64_t yrow = active ? std::int64_t(M.perm[tile->row0 + lane]) * M.ldy + p0 : 0;
     double total[KB];
     wave_rows<KB, FP32, W>(M, nullptr, tile->split2, wave, p0, kk, R,
                            lds + wave * wave_lds_bytes<KB>(), total);
     double* y = M.y;
     combine_rows<KB, W>(total, base, lds, wave, lane, [&](int pc, double a0, double a1) {
+        if (!early) {
+            a0 += pass1_rows(pc);
+            if (pc + 1 < KB) a1 += pass1_rows(pc + 1);
+        }
         if (active) {
             if (pc < kk) y[yrow + pc] = a0;
             if (pc + 1 < kk) y[yrow + pc + 1] = a1;
@@ -742,10 +833,15 @@ __device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t,
     });
 }
 
-template <int KB, bool FP32, int W>
-__global__ __launch_bounds__(64 * W) void apply_pass1(MemberView M) {
-    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
-    pass1_tile<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
+// Pass-1 waves per SIMD the register allocation must allow, by right-hand
+// sides per pass.
+constexpr int pass1_occupancy(int kb) { return kb <= 1 ? 4 : kb <= 8 ? 2 : 1; }
+
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kWaves / kPass1Groups, pass1_occupancy(KB)) void apply_pass1(
+    MemberView M) {
+    __shared__ __align__(16) std::uint8_t lds[kWaves / kPass1Groups * wave_lds_bytes<KB>()];
+    pass1_group<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
 template <int KB, bool FP32, int W>
@@ -778,17 +874,17 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
     return M;
 }
 
-template <int KB, bool FP32, int W>
-__global__ __launch_bounds__(64 * W) void batch_pass1(const MemberView* __restrict__ members,
-                                                      const std::uint32_t* __restrict__ first,
-                                                      std::uint32_t count, const double* x,
-                                                      double* y, double* zp) {
-    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
-    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kWaves / kPass1Groups, pass1_occupancy(KB)) void batch_pass1(
+    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
+    std::uint32_t count, const double* x, double* y, double* zp) {
+    __shared__ __align__(16) std::uint8_t lds[kWaves / kPass1Groups * wave_lds_bytes<KB>()];
+    const std::uint32_t g = blockIdx.x / kPass1Groups;
+    const std::uint32_t m = locate_member(first, count, g);
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
-    pass1_tile<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
+    pass1_group<KB, FP32>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
 }
 
 template <int KB, bool FP32, int W>
@@ -839,6 +935,7 @@ MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, 
     M.ldy = ldy;
     M.k = k;
     M.tiles_count = s.n_tiles;
+    M.order = std::uint32_t(s.order);
     M.z_rows = s.z_rows;
     return M;
 }
@@ -853,6 +950,15 @@ hipEvent_t* profile_slot(const DeviceStore& s) {
 // Waves per workgroup: fewer waves with more registers each as the
 // right-hand sides per pass grow (the wave lists are cut for 16 waves; a
 // wave of a smaller workgroup takes 16/W consecutive lists).
+// Right-hand sides per pass for an apply of k: the tuned choice when one
+// was set for this k (qh_store_set_rhs_block), else k rounded up to a power
+// of two, at most 16.
+int default_rhs_block(std::int64_t k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 16; }
+int rhs_block(const DeviceStore& s, std::int64_t k) {
+    const auto it = s.rhs_block.find(k);
+    return it == s.rhs_block.end() ? default_rhs_block(k) : it->second;
+}
+
 constexpr int waves_for(int kb) { return kb >= 16 ? 4 : kb >= 2 ? 8 : 16; }
 
 template <int KB, bool FP32>
@@ -863,8 +969,8 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
-    hipLaunchKernelGGL((apply_pass1<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
-                       stream, M);
+    hipLaunchKernelGGL((apply_pass1<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
+                       dim3(64 * kWaves / kPass1Groups), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
     hipLaunchKernelGGL((apply_pass2<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
@@ -879,8 +985,9 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
     constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
-    hipLaunchKernelGGL((batch_pass1<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
-                       stream, c.members, c.first_tile, count, x, y, zp);
+    hipLaunchKernelGGL((batch_pass1<KB, FP32>), dim3(c.total_tiles * kPass1Groups, passes),
+                       dim3(64 * kWaves / kPass1Groups), 0, stream, c.members, c.first_tile, count,
+                       x, y, zp);
     hip_check(hipGetLastError(), "batched pass 1 launch");
     hipLaunchKernelGGL((batch_pass2<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
                        stream, c.members, c.first_tile, count, x, y, zp);
@@ -902,11 +1009,13 @@ void dispatch_batch_class(const BatchClass& c, const double* x, double* y, doubl
 template <bool FP32>
 void dispatch_rhs(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
                   std::int64_t ldy, double* zp, hipStream_t st) {
-    if (k <= 1) return launch_apply<1, FP32>(s, x, ldx, k, y, ldy, zp, st);
-    if (k <= 2) return launch_apply<2, FP32>(s, x, ldx, k, y, ldy, zp, st);
-    if (k <= 4) return launch_apply<4, FP32>(s, x, ldx, k, y, ldy, zp, st);
-    if (k <= 8) return launch_apply<8, FP32>(s, x, ldx, k, y, ldy, zp, st);
-    return launch_apply<16, FP32>(s, x, ldx, k, y, ldy, zp, st);
+    switch (rhs_block(s, k)) {
+        case 1: return launch_apply<1, FP32>(s, x, ldx, k, y, ldy, zp, st);
+        case 2: return launch_apply<2, FP32>(s, x, ldx, k, y, ldy, zp, st);
+        case 4: return launch_apply<4, FP32>(s, x, ldx, k, y, ldy, zp, st);
+        case 8: return launch_apply<8, FP32>(s, x, ldx, k, y, ldy, zp, st);
+        default: return launch_apply<16, FP32>(s, x, ldx, k, y, ldy, zp, st);
+    }
 }
 
 template <class T> T* upload(const std::vector<T>& host, std::uint64_t& bytes) {
@@ -1485,7 +1594,7 @@ void destroy_device_store(DeviceStore* s) {
 }
 
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k) {
-    return (s.z_rows + s.order) * std::uint64_t(k) * sizeof(double);
+    return (s.z_rows + kPass1Groups * s.order) * std::uint64_t(k) * sizeof(double);
 }
 
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,
@@ -1499,6 +1608,10 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
             "unknown precision mode");
     const std::uint64_t need = apply_workspace_bytes(s, k);
     require(workspace == nullptr || workspace_bytes >= need, "workspace is too small");
+    // the kernels address x and the workspace with 32-bit byte offsets
+    require(((s.order - 1) * std::uint64_t(ldx) + std::uint64_t(k)) * sizeof(double) < kOutOfRange &&
+                need < kOutOfRange,
+            "right-hand sides too large for one apply (split the columns)");
     DeviceScope scope(s.device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     double* zp = static_cast<double*>(workspace);
@@ -1582,11 +1695,13 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
 // ---------------------------------------------------------------------------
 // Batches
 
-namespace {
-int rhs_class(std::int64_t k) {
-    return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 16;
+void set_rhs_block(DeviceStore& s, std::int64_t k, int kb) {
+    require(k > 0, "right-hand-side shape mismatch");
+    require(kb == 0 || kb == 1 || kb == 2 || kb == 4 || kb == 8 || kb == 16,
+            "right-hand sides per pass must be 1, 2, 4, 8 or 16 (0 = default)");
+    if (kb == 0) s.rhs_block.erase(k);
+    else s.rhs_block[k] = kb;
 }
-}  // namespace
 
 Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std::uint64_t count) {
     require(count > 0 && stores && k, "empty batch");
@@ -1615,7 +1730,7 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
             std::vector<MemberView> views;
             std::vector<std::uint32_t> ft{0};
             for (std::uint64_t i = 0; i < count; ++i) {
-                if (rhs_class(b->k[i]) != kb) continue;
+                if (rhs_block(*b->stores[i], b->k[i]) != kb) continue;
                 const DeviceStore& s = *b->stores[i];
                 // x, y and the workspace as element offsets from the call's bases
                 views.push_back(member_view(
@@ -1661,6 +1776,8 @@ void batch_apply(const Batch& b, const double* x, double* y, int precision, void
             "unknown precision mode");
     const std::uint64_t need = b.z_total * sizeof(double);
     require(workspace == nullptr || workspace_bytes >= need, "workspace is too small");
+    require(b.total * sizeof(double) < kOutOfRange && need < kOutOfRange,
+            "batch right-hand sides too large for one launch");
     DeviceScope scope(b.device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     double* zp = static_cast<double*>(workspace);

@@ -27,14 +27,16 @@
 // When a tile starts on a multiple of 32 codes its stream words are the
 // reference's words verbatim.
 //
-// An apply is two launches of one workgroup per row tile:
-//  pass 1  the tile's kDense and kFused items, whose row sums go to the
-//          workspace, and a share of the kRight items;
-//  pass 2  the tile's kLeft items plus the pass-1 row sums, written to y
-//          through the permutation.
+// An apply is two launches:
+//  pass 1  four workgroups per row tile, each running four of the tile's 16
+//          item runs: the tile's kDense and kFused items, whose row sums go
+//          to the workspace, and a share of the kRight items;
+//  pass 2  one workgroup per row tile: the tile's kLeft items plus the
+//          pass-1 row sums, written to y through the permutation.
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <vector>
 
 #include "../../include/qhmat_hip.h"
@@ -42,7 +44,8 @@
 namespace qhamd {
 
 constexpr int kTile = 64;          // rows per row tile, columns per column tile
-constexpr int kWaves = 16;         // waves per workgroup (wave lists are cut for 16)
+constexpr int kWaves = 16;         // item runs per tile and pass
+constexpr int kPass1Groups = 4;    // pass-1 workgroups per tile (4 runs each)
 constexpr int kItemQuads = 256;    // 16-byte groups of data per item (4 KiB)
 constexpr int kItemComponents = 64;  // components per item
 constexpr int kDenseColumns = 8;     // columns per kDense item (64 x 8 FP64 = 4 KiB)
@@ -90,10 +93,13 @@ struct MemberView {
     const TileDev* tiles;
     const double* x;
     double* y;
-    double* zp;     // workspace: z_rows x k partial sums, then order x k row sums
+    double* zp;     // workspace: z_rows x k partial sums, then kPass1Groups sets of
+                    // order x k row sums
     std::int64_t ldx, ldy;
     std::int32_t k;
     std::uint32_t tiles_count;  // row tiles
+    std::uint32_t order;
+    std::uint32_t pad;
     std::uint64_t z_rows;       // first workspace row of the row sums
 };
 
@@ -117,7 +123,7 @@ struct DeviceStore {
     // sizes
     std::uint64_t n_words = 0, n_comp = 0;
     std::uint32_t n_tiles = 0, n_items = 0;
-    std::uint64_t z_rows = 0;  // workspace = (z_rows + order) * k doubles
+    std::uint64_t z_rows = 0;  // workspace = (z_rows + kPass1Groups * order) * k doubles
     // host mirrors needed by entry points
     std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;  // per low-rank block
@@ -125,6 +131,9 @@ struct DeviceStore {
     std::vector<std::uint32_t> ranks;                              // per low-rank block
     qh_store_stats stats{};
     std::uint64_t device_bytes = 0;
+    // Right-hand sides per pass chosen for a right-hand-side count
+    // (qh_store_set_rhs_block, e.g. by the autotuner); others use the default.
+    std::map<std::int64_t, int> rhs_block;
     // Optional per-pass event timing (qh_profile_begin / qh_profile_end).
     struct Profile {
         std::vector<void*> events;  // hipEvent_t, 3 per apply
@@ -201,6 +210,7 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
                   void* stream);
 int device_count();
 void profile_begin(DeviceStore& s, std::uint64_t max_applies);
+void set_rhs_block(DeviceStore& s, std::int64_t k, int kb);
 qh_profile profile_end(DeviceStore& s);
 
 // Appends codes [first, first+count) of one component stream of width `bits`

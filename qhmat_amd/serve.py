@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import collections
 import queue
+import sys
 import threading
 import time
 from concurrent.futures import Future
@@ -43,6 +44,7 @@ class _Request:
     ready: object  # torch.cuda.Event recorded after x was written
     k: int
     arrival: float
+    finite: object = None  # device bool for a device-tensor x, checked on completion
     future: Future = field(default_factory=Future)
 
 
@@ -113,15 +115,21 @@ class Server:
         res = self._stores.get(name)
         if res is None:
             raise KeyError(f"store {name!r} is not resident")
+        finite = None
         if isinstance(x, np.ndarray):
-            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64))
-        x = x.to(device=f"cuda:{self.device}", dtype=torch.float64)
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            if not np.isfinite(x).all():
+                raise ValueError("nonfinite right-hand side")
+            x = torch.from_numpy(x).to(device=f"cuda:{self.device}")
+        else:
+            x = x.to(device=f"cuda:{self.device}", dtype=torch.float64)
+            # checked on the device, without a host round trip; the request's
+            # future raises when its pass completes
+            finite = torch.isfinite(x).all()
         x = x.reshape(res.order, -1)
-        if not bool(torch.isfinite(x).all()):
-            raise ValueError("nonfinite right-hand side")
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
-        req = _Request(name, x, ready, int(x.shape[1]), time.perf_counter())
+        req = _Request(name, x, ready, int(x.shape[1]), time.perf_counter(), finite)
         with self._cv:
             self._queue.append(req)
             self._cv.notify()
@@ -219,7 +227,10 @@ class Server:
             for reqs, y, cost in work:
                 col = 0
                 for r in reqs:
-                    r.future.set_result(y[:, col:col + r.k])
+                    if r.finite is not None and not bool(r.finite):
+                        r.future.set_exception(ValueError("nonfinite right-hand side"))
+                    else:
+                        r.future.set_result(y[:, col:col + r.k])
                     col += r.k
                     self.latencies.append(now - r.arrival)
                     self.completed += 1
@@ -258,18 +269,27 @@ def poisson_load(server: Server, names: list[str], orders: dict[str, int], rate:
           for nm in names for k in ks}
     torch.cuda.synchronize()
     futures = []
-    t0 = time.perf_counter()
-    s_name, s_k = None, None
-    for i, at in enumerate(arrivals):
-        if i % session == 0:
-            s_name = names[int(rng.integers(len(names)))]
-            s_k = int(ks[int(rng.integers(len(ks)))])
-        while time.perf_counter() - t0 < at:
-            pass
-        futures.append(server.submit(s_name, xs[(s_name, s_k)]))
-    for f in futures:
-        f.result()
-    elapsed = time.perf_counter() - t0
+    # The generator sleeps until each arrival (a busy-wait would hold the
+    # interpreter lock from the dispatcher), with a short switch interval so
+    # the server threads run promptly.
+    interval = sys.getswitchinterval()
+    sys.setswitchinterval(5e-5)
+    try:
+        t0 = time.perf_counter()
+        s_name, s_k = None, None
+        for i, at in enumerate(arrivals):
+            if i % session == 0:
+                s_name = names[int(rng.integers(len(names)))]
+                s_k = int(ks[int(rng.integers(len(ks)))])
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            futures.append(server.submit(s_name, xs[(s_name, s_k)]))
+        for f in futures:
+            f.result()
+        elapsed = time.perf_counter() - t0
+    finally:
+        sys.setswitchinterval(interval)
     out = {"offered_rate": rate, "requests": len(futures), "seconds": elapsed,
            "throughput_rps": len(futures) / elapsed,
            "passes": server.passes}

@@ -80,3 +80,22 @@ def test_requests_wait_for_memory(natives):
         for f in futures:
             assert torch.equal(f.result(timeout=60), want)
         assert server.passes == 5
+
+
+def test_nonfinite_requests_raise(natives):
+    import torch
+
+    from qhmat_amd.serve import Server
+
+    n = natives["line"].order
+    bad = np.ones((n, 2))
+    bad[3, 1] = np.nan
+    with Server(hbm_budget=1 << 32) as server:
+        server.admit("line", natives["line"])
+        with pytest.raises(ValueError, match="nonfinite"):
+            server.submit("line", bad)  # host array: checked before queueing
+        fut = server.submit("line", torch.from_numpy(bad).cuda())  # device: checked on completion
+        ok = server.submit("line", torch.ones(n, 1, dtype=torch.float64, device="cuda"))
+        with pytest.raises(ValueError, match="nonfinite"):
+            fut.result(timeout=60)
+        assert ok.result(timeout=60).shape == (n, 1)
