@@ -35,6 +35,23 @@ FP64_PEAK_TFLOPS = 78.6
 T0 = time.time()
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "apply_square_exp_16384_hbm_traffic.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 256 MiB copy), or None
+    when the summary has no line for it."""
+    try:
+        lines = TRAFFIC_PROFILE.read_text().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        if line.startswith("traffic ") and line[len("traffic "):].rsplit(" ", 1)[0] == kernel:
+            return int(line.rsplit(" ", 1)[1])
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
@@ -338,6 +355,7 @@ def main() -> int:
     dom_ms = r1["passes_ms"][dominant]
     dom_bytes = b1[f"pass{dominant}"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    dom_kernel = ["apply_pass1<1, true>", "apply_pass2<1, true, 16>"][dominant]
     line = {
         "metric": "H-matvec GB/s (% HBM peak) FP32 vs FP64 at rel err; served p50/p99 ms",
         "value": round(head["gbps"], 2),
@@ -359,8 +377,8 @@ def main() -> int:
         "pct_hbm_peak": round(100 * head["gbps"] / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": ["apply_pass1", "apply_pass2"][dominant],
+                     "traffic": measured_traffic(dom_kernel),
+                     "kernel": dom_kernel,
                      "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
     }
     for k in args.rhs:

@@ -5,21 +5,22 @@
 // deterministic (fixed summation order, one writer per output element, no
 // floating-point atomics):
 //
-//  pass 1  one workgroup per row tile: its waves sum the tile's diagonal
-//          (kDense) and fused (kFused: z and U z inside the item)
-//          contributions, added in a fixed tree into the workspace, and
-//          stream a share of the kRight items: partial sums
+//  pass 1  four workgroups of four waves per row tile, each wave running one
+//          of the tile's 16 item runs: the tile's diagonal (kDense) and
+//          fused (kFused: z and U z inside the item) contributions, summed
+//          in a fixed tree per workgroup into one of four row-sum sets of
+//          the workspace, and a share of the kRight items: partial sums
 //          z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c] into the workspace.
 //  pass 2  one workgroup per row tile: the kLeft items (U z with z summed
-//          over the block's column tiles), added to the pass-1 rows and
-//          written once to y[perm[row]].
+//          over the block's column tiles), added to the four pass-1 row sums
+//          and written once to y[perm[row]].
 //
-// Every wave walks a contiguous list of items with the next item's data
-// (<= 4 KiB, four 16-byte loads per lane), component widths, scales and
-// inputs in flight while the current item is decoded from LDS: the record
-// list is read once (lanes = items), the permutation rows of the item after
-// next are requested with the next item's data, so no load waits on another
-// inside the loop.
+// Every wave walks a contiguous list of items with the data of the next two
+// items (<= 4 KiB each, four 16-byte buffer loads per lane), their component
+// widths, scales and inputs in flight while the current item is decoded from
+// LDS.  The loads are buffer-resource loads: a lane past an item's end reads
+// zero instead of branching, and a wave past its run's end fetches an empty
+// item, so the loop has no load under a branch and no load waits on another.
 //
 // Decoding: a code is v_alignbit_b32 of two staged words and a sign-extending
 // shift pair, branch-free for every width.  FP64 mode computes in FP64
