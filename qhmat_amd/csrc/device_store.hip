@@ -89,17 +89,6 @@ __host__ __device__ __forceinline__ std::uint32_t stream_words(std::uint32_t cou
     return (count * bits + 31u) >> 5;
 }
 
-// Lane j's value, broadcast to the wave (v_readlane_b32; j is wave-uniform).
-__device__ __forceinline__ float lane_value(float v, std::uint32_t j) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ double lane_value(double v, std::uint32_t j) {
-    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
-    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), j));
-    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), j));
-    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
 // LDS written by some lanes of a wave and read by others: keep the compiler
 // from moving accesses across this point and wait for the LDS queue.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -129,12 +118,17 @@ typedef std::uint32_t quad_t __attribute__((ext_vector_type(4)));
 
 template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
 
-// LDS of one wave: the current item's data plus slack that lanes decoding
-// past the end of a stream may read (64 codes of 32 bits and the word after),
-// then the item's inputs (x rows of a diagonal item, or the workspace
-// partial sums of kLeft).
+// LDS of one wave: a head word that decoding may read below the first
+// stream, the current item's data plus slack that lanes decoding past the end
+// of a stream may read (64 codes of 32 bits and the word after), then the
+// item's inputs (x rows of its columns, or the workspace partial sums of
+// kLeft), KB values per lane, which also take the item's rows at its end,
+// then the scaled inputs of 16 components (zt).
 constexpr int kStageBytes = kItemQuads * 16 + 272;
-template <int KB> constexpr int wave_lds_bytes() { return kStageBytes + kTile * KB * 8; }
+constexpr int kLdsHead = 16;  // readable words before the stage (decode_end)
+template <int KB> constexpr int wave_lds_bytes() {
+    return kLdsHead + kStageBytes + kTile * KB * 8 + 16 * KB * 8;
+}
 
 // Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
 // items of the group).
@@ -235,6 +229,85 @@ __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::i
         F.in[p] = load_f64(r, (mine && p < kk) ? (row + std::uint32_t(p0 + p)) * 8u : kOutOfRange);
 }
 
+// Per-component uniform fields packed in one lane value: stream offset in
+// words (< 2^16) and width.  Lanes past the item's components hold a dummy
+// (offset = end of the streams, 1 bit) whose input is zero, so loops over
+// whole groups of components need no tests.
+__device__ __forceinline__ std::uint32_t pack_meta(bool comp, std::uint32_t off, std::uint32_t bits) {
+    return off | ((comp ? bits : 1u) << 16);
+}
+
+// Matrix cores.  v_mfma_f32_16x16x4f32 and v_mfma_f64_16x16x4f64 form
+// D += A B with A 16 x 4 and B 4 x 16: lane l = 16 q + r supplies A[r][q] and
+// B[q][r], and slot t of its result holds D[4q + t][r] (FP32) or D[4t + q][r]
+// (FP64).  In FP64 mode lane r supplies row a_row(r) of A instead, which puts
+// D[4q + t][r] in slot t as well, so both modes share one result layout.
+// Each result element is its own dot product: column r of D depends on
+// column r of B only, so a right-hand side's arithmetic does not depend on
+// the others in its pass.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Right-hand sides per pass from which the matrix cores take the codes (below,
+// one code's products are too few to fill them and the vector path is faster).
+constexpr int kMatrixMinKB = 8;
+
+template <bool FP32> struct Mma;
+template <> struct Mma<true> {
+    using acc = f32x4;
+    static __device__ __forceinline__ acc zero() { return acc{0.0f, 0.0f, 0.0f, 0.0f}; }
+    static __device__ __forceinline__ acc mad(float a, float b, acc d) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+    }
+};
+template <> struct Mma<false> {
+    using acc = f64x4;
+    static __device__ __forceinline__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc mad(double a, double b, acc d) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+    }
+};
+template <bool FP32> __host__ __device__ __forceinline__ int a_row(int r) {
+    return FP32 ? r : 4 * (r & 3) + (r >> 2);
+}
+
+// The b-bit code that ends just below bit E of the staged words (one readable
+// word is kept before the stage): the 32 bits [E - 32, E) from the word pair
+// (v_alignbit_b32), shifted arithmetically right by sh = 32 - b, which
+// sign-extends it (b = 32 included).  Bits below the code's stream are
+// shifted out, so no branch and no mask.
+__device__ __forceinline__ int decode_end(const std::uint32_t* stage, std::uint32_t E,
+                                          std::uint32_t sh) {
+    const std::uint32_t* w = stage + (E >> 5);
+    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-1], E)) >> sh;
+}
+
+// A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
+// exact in FP32, and a code of small magnitude stays in l (no cancellation
+// between the halves).
+__device__ __forceinline__ void split_code(int code, float& h, float& l) {
+    // (code - l) / 2^16 without leaving 32 bits: the arithmetic high half,
+    // plus one when the low half is negative
+    h = float((code >> 16) + ((code >> 15) & 1));
+    l = float(static_cast<int>(__builtin_amdgcn_sbfe(code, 0u, 16u)));
+}
+
+// ---------------------------------------------------------------------------
+// Narrow passes (KB < kMatrixMinKB): vector ALUs.  Lanes are the item's rows
+// (left factors, diagonal columns) or columns (right factors, summed over the
+// lanes by a butterfly of lane swaps).
+
+// Lane j's value, broadcast to the wave (v_readlane_b32; j is wave-uniform).
+__device__ __forceinline__ float lane_value(float v, std::uint32_t j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_value(double v, std::uint32_t j) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), j));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), j));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // Code `idx` of a stream whose first word is at `base` (LDS): the word pair
 // holding the code (v_alignbit_b32), then a shift pair that sign-extends the
 // b-bit field (also for b = 32).  No branches: lanes past the end of a
@@ -248,13 +321,6 @@ __device__ __forceinline__ int decode_lane(const std::uint32_t* base, std::uint3
     return static_cast<int>(raw << sh) >> sh;
 }
 
-// Per-component uniform fields packed in one lane value: stream offset in
-// words (< 2^16) and width.  Lanes past the item's components hold a dummy
-// (offset = end of the streams, 1 bit) whose input is zero, so loops over
-// whole groups of components need no tests.
-__device__ __forceinline__ std::uint32_t pack_meta(bool comp, std::uint32_t off, std::uint32_t bits) {
-    return off | ((comp ? bits : 1u) << 16);
-}
 
 // Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
 // lane-half swaps (v_permlane32_swap / v_permlane16_swap).
@@ -365,15 +431,6 @@ __host__ __device__ __forceinline__ bool block_writer(int lane) {
     return (lane & (N == 16 ? 3 : N == 8 ? 7 : 15)) == 0;
 }
 
-// A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
-// exact in FP32, and a code of small magnitude stays in l (no cancellation
-// between the halves).
-__device__ __forceinline__ void split_code(int code, float& h, float& l) {
-    // (code - l) / 2^16 without leaving 32 bits: the arithmetic high half,
-    // plus one when the low half is negative
-    h = float((code >> 16) + ((code >> 15) & 1));
-    l = float(static_cast<int>(__builtin_amdgcn_sbfe(code, 0u, 16u)));
-}
 
 // Row contributions of four left-factor components (lanes are rows); zs[u]
 // is component u's scaled input, zero for padding components.  In FP32 mode
@@ -381,7 +438,7 @@ __device__ __forceinline__ void split_code(int code, float& h, float& l) {
 // bits (groups of four start at multiples of four, whatever the
 // right-hand-side count).
 template <int KB, bool FP32>
-__device__ __forceinline__ void left_four(const std::uint32_t* stage, const std::uint32_t (&meta)[4],
+__device__ __forceinline__ void valu_left_four(const std::uint32_t* stage, const std::uint32_t (&meta)[4],
                                           bool wide, int lane, const real_t<FP32> (&zs)[4][KB],
                                           double (&acc)[KB], float (&blk)[KB]) {
     int code[4];
@@ -417,11 +474,11 @@ __device__ __forceinline__ void left_four(const std::uint32_t* stage, const std:
 
 // Left-factor rows of components [j0, jend) of an item, four at a time
 // (j0 a multiple of 4 and jend <= 64, so every lane index stays below 64):
-// meta held one per lane (lanes = components), zs_of(j, p) the scaled input
+// meta held one per lane (lanes = components), zs_of(j, u, p) the scaled input
 // of component j for right-hand side p (lane-uniform), zero for components
 // past jend, whose dummy meta decodes in-bounds 1-bit codes.
 template <int KB, bool FP32, class ZsOf>
-__device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint32_t lmeta_l,
+__device__ __forceinline__ void valu_left_rows(const std::uint32_t* stage, std::uint32_t lmeta_l,
                                           std::uint32_t j0, std::uint32_t jend, int lane,
                                           ZsOf zs_of, double (&acc)[KB], float (&blk)[KB]) {
     using Real = real_t<FP32>;
@@ -434,9 +491,9 @@ __device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint3
             meta[u] = __builtin_amdgcn_readlane(lmeta_l, j + u);
             wide = wide || (meta[u] >> 16) > 24u;
 #pragma unroll
-            for (int p = 0; p < KB; ++p) zs[u][p] = zs_of(j + u, p);
+            for (int p = 0; p < KB; ++p) zs[u][p] = zs_of(j, u, p);
         }
-        left_four<KB, FP32>(stage, meta, wide, lane, zs, acc, blk);
+        valu_left_four<KB, FP32>(stage, meta, wide, lane, zs, acc, blk);
     }
 }
 
@@ -450,7 +507,7 @@ __device__ __forceinline__ void left_rows(const std::uint32_t* stage, std::uint3
 // The blocks depend only on the item's component count, so a component's
 // rounding is the same for every right-hand-side count.
 template <int N, int KB, bool FP32>
-__device__ __forceinline__ void right_block(const std::uint32_t* stage, std::uint32_t rmeta_l,
+__device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std::uint32_t rmeta_l,
                                             std::uint32_t j0, std::uint32_t count, int lane,
                                             const real_t<FP32> (&xl)[KB], double (&tot)[KB]) {
     using Real = real_t<FP32>;
@@ -491,7 +548,7 @@ __device__ __forceinline__ void right_block(const std::uint32_t* stage, std::uin
 // then (kRight) their workspace rows or (kFused) the scaled inputs and the
 // left streams of the same components.
 template <int N, int KB, bool FP32>
-__device__ __forceinline__ void right_then_left(const MemberView& M, const Item& f,
+__device__ __forceinline__ void valu_right_then_left(const MemberView& M, const Item& f,
                                                 const std::uint32_t* stage, std::uint32_t rmeta_l,
                                                 std::uint32_t lmeta_l, double scale_l,
                                                 std::uint32_t j0, int lane, int p0, int kk,
@@ -499,7 +556,7 @@ __device__ __forceinline__ void right_then_left(const MemberView& M, const Item&
                                                 float (&blk)[KB]) {
     using Real = real_t<FP32>;
     double tot[KB];
-    right_block<N, KB, FP32>(stage, rmeta_l, j0, f.count, lane, xl, tot);
+    valu_right_block<N, KB, FP32>(stage, rmeta_l, j0, f.count, lane, xl, tot);
     const std::uint32_t jl = j0 + std::uint32_t(block_slot<N>(lane));
     if (f.kind == kRight) {
         if (block_writer<N>(lane) && jl < f.count) {
@@ -514,59 +571,296 @@ __device__ __forceinline__ void right_then_left(const MemberView& M, const Item&
     Real zs_lane[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zs_lane[p] = jl < f.count ? Real(sc * tot[p]) : Real(0);
-    left_rows<KB, FP32>(stage, lmeta_l, j0, min(j0 + N, f.count), lane,
-                        [&](std::uint32_t j, int p) {
-                            return lane_value(zs_lane[p], block_lane<N>(int(j - j0)));
+    valu_left_rows<KB, FP32>(stage, lmeta_l, j0, min(j0 + N, f.count), lane,
+                        [&](std::uint32_t j, int u, int p) {
+                            return lane_value(zs_lane[p], block_lane<N>(int(j + u - j0)));
                         },
                         acc, blk);
 }
 
-// The registers of an item that outlive its staging: component widths and
-// scale products (lanes = components) and x rows or partial sums.
-template <int KB> struct Staged {
-    std::uint32_t bits_l;
-    double scale_l;
-    double in[KB];
+
+// ---------------------------------------------------------------------------
+// Wide passes (KB >= kMatrixMinKB): matrix cores.
+
+// Per-item rows on the matrix cores: slot t of d[rb] in lane 16q + r holds
+// row 16 rb + 4q + t of the item's tile for right-hand side r mod KB.
+template <bool FP32> struct ItemRows {
+    typename Mma<FP32>::acc d[4];
+    __device__ __forceinline__ ItemRows() {
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) d[rb] = Mma<FP32>::zero();
+    }
 };
 
-// Moves a fetched item into the wave's LDS (data, and the x rows of a
-// diagonal item or the partial sums of a kLeft item) and keeps the rest.
+// Left-factor rows of components [g0, g0 + 16) of an item (g0 a multiple of
+// 16) on the matrix cores, one MFMA per 16 rows x 4 components: lane 16q + r
+// decodes row 16 rb + a_row(r) of component g0 + 4s + q and supplies that
+// component's scaled input for right-hand side r mod KB from the LDS table
+// zt (zt[(c - g0) KB + p], zero past the item's components, whose dummy
+// streams decode in bounds).  Rows past the tile's decode neighbouring bits,
+// and only their own results, which are never used.  WIDE (FP32 mode, an
+// item with a code wider than 24 bits): every code is split (split_code), h
+// against 2^16 zs and l against zs, into the same sums.
+template <int KB, bool FP32, bool WIDE>
+__device__ __forceinline__ void left_group(const std::uint32_t* stage, const real_t<FP32>* zt,
+                                           std::uint32_t lmeta_l, std::uint32_t g0, int lane,
+                                           ItemRows<FP32>& rows) {
+    using Mm = Mma<FP32>;
+    using Real = real_t<FP32>;
+    const int q = lane >> 4, r = lane & 15;
+    const std::uint32_t row1 = std::uint32_t(a_row<FP32>(r)) + 1u;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const std::uint32_t c = g0 + 4u * std::uint32_t(s) + std::uint32_t(q);  // <= 63
+        const std::uint32_t meta = std::uint32_t(__shfl(int(lmeta_l), int(c), 64));
+        const std::uint32_t bits = meta >> 16, sh = 32u - bits;
+        const std::uint32_t E0 = (meta & 0xFFFFu) * 32u + row1 * bits, step = 16u * bits;
+        const Real zs = zt[(4 * s + q) * KB + (r & (KB - 1))];
+        if constexpr (!WIDE) {
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                rows.d[rb] = Mm::mad(Real(decode_end(stage, E0 + rb * step, sh)), zs, rows.d[rb]);
+        } else {
+            const float zh = float(zs) * 65536.0f;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                float h, l;
+                split_code(decode_end(stage, E0 + rb * step, sh), h, l);
+                rows.d[rb] = Mma<true>::mad(h, zh, rows.d[rb]);
+                rows.d[rb] = Mma<true>::mad(l, float(zs), rows.d[rb]);
+            }
+        }
+    }
+}
+
+// Diagonal-block columns on the FP64 matrix cores: lane 16q + r supplies
+// value (row 16 rb + a_row(r), column 4s + q) of the staged column-major
+// R x cnt block (zero for columns past cnt, whose x is zero too) and x of
+// that column for right-hand side r mod KB.
 template <int KB>
-__device__ __forceinline__ Staged<KB> stage_item(const Item& f, const Fetch<KB>& F, int lane,
-                                                 std::uint8_t* wave_lds) {
-    quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds);
-    double* xs = reinterpret_cast<double*>(wave_lds + kStageBytes);
+__device__ __forceinline__ void dense_rows(const double* dv, const double* xs, std::uint32_t R,
+                                           std::uint32_t cnt, int lane, ItemRows<false>& rows) {
+    const int q = lane >> 4, r = lane & 15;
+    const std::uint32_t row = std::uint32_t(a_row<false>(r));
+#pragma unroll
+    for (int s = 0; s < kDenseColumns / 4; ++s) {
+        const std::uint32_t c = 4u * std::uint32_t(s) + std::uint32_t(q);
+        const double xv = xs[c * KB + (r & (KB - 1))];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const double a = c < cnt ? dv[c * R + 16u * std::uint32_t(rb) + row] : 0.0;
+            rows.d[rb] = Mma<false>::mad(a, xv, rows.d[rb]);
+        }
+    }
+}
+
+// Adds an item's rows to the wave's row sums (lanes = rows) through the LDS
+// tile T[row][KB] (the item's input area, no longer read by then), in the
+// precision the rows were summed in.
+template <int KB, bool FP32>
+__device__ __forceinline__ void add_item_rows(const ItemRows<FP32>& rows, void* tile, int lane,
+                                              double (&acc)[KB]) {
+    using Real = real_t<FP32>;
+    Real* T = static_cast<Real*>(tile);
+    const int q = lane >> 4, r = lane & 15;
+    if (r < KB) {
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) T[(16 * rb + 4 * q + t) * KB + r] = rows.d[rb][t];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < KB; ++p) acc[p] += double(T[lane * KB + p]);
+}
+
+// Right-factor sums of components [g0, g0 + 16) of a kRight or kFused item
+// on the matrix cores, one MFMA per 4 x 16 codes: z[t] in lane 16q + r is
+//   sum over the item's 64 columns c of code(c, g0 + 4q + t) x[c][r mod KB]
+// in FP64.  MFMA m (0..15) puts columns 16q + m in its k slots: each lane
+// decodes its own component's code of column 16q + m (stepping the end bit
+// by the width) and reads that column's x from LDS.  Columns past the item's
+// hold zero x; components past its count decode the dummy stream and their
+// sums are never used.  WIDE (FP32 mode, an item with a code wider than 24
+// bits): every code is split into exact halves (split_code) summed apart and
+// joined in FP64.  Both depend only on the item, so a component's rounding
+// is the same for every right-hand-side count.
+template <int KB, bool FP32, bool WIDE>
+__device__ __forceinline__ void right_group(const std::uint32_t* stage, const double* xs,
+                                            std::uint32_t rmeta_l, std::uint32_t g0, int lane,
+                                            double (&z)[4]) {
+    using Mm = Mma<FP32>;
+    using Real = real_t<FP32>;
+    const int q = lane >> 4, r = lane & 15;
+    const std::uint32_t j = g0 + std::uint32_t(a_row<FP32>(r));  // <= 63
+    const std::uint32_t meta = std::uint32_t(__shfl(int(rmeta_l), int(j), 64));
+    const std::uint32_t bits = meta >> 16, sh = 32u - bits;
+    std::uint32_t E = (meta & 0xFFFFu) * 32u + (16u * std::uint32_t(q) + 1u) * bits;
+    const double* xcol = xs + 16 * q * KB + (r & (KB - 1));
+    if constexpr (!WIDE) {
+        typename Mm::acc d = Mm::zero();
+#pragma unroll 4
+        for (int m = 0; m < 16; ++m) {
+            const int code = decode_end(stage, E, sh);
+            E += bits;
+            d = Mm::mad(Real(code), Real(xcol[m * KB]), d);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) z[t] = double(d[t]);
+    } else {
+        f32x4 dh = Mma<true>::zero(), dl = Mma<true>::zero();
+#pragma unroll 4
+        for (int m = 0; m < 16; ++m) {
+            const int code = decode_end(stage, E, sh);
+            E += bits;
+            float h, l;
+            split_code(code, h, l);
+            const float xv = float(xcol[m * KB]);
+            dh = Mma<true>::mad(h, xv, dh);
+            dl = Mma<true>::mad(l, xv, dl);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) z[t] = double(dh[t]) * 65536.0 + double(dl[t]);
+    }
+}
+
+// kFused components [g0, g0 + 16): right-factor sums, their scaled inputs
+// through the LDS table zt, and the left streams of the same components.
+template <int KB, bool FP32, bool WIDE>
+__device__ __forceinline__ void fused_group(const Item& f, const std::uint32_t* stage,
+                                            const double* xs, real_t<FP32>* zt,
+                                            std::uint32_t rmeta_l, std::uint32_t lmeta_l,
+                                            double scale_l, std::uint32_t g0, int lane,
+                                            ItemRows<FP32>& rows) {
+    using Real = real_t<FP32>;
+    const int q = lane >> 4, r = lane & 15;
+    double z[4];
+    right_group<KB, FP32, WIDE>(stage, xs, rmeta_l, g0, lane, z);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const std::uint32_t c = g0 + 4u * std::uint32_t(q) + std::uint32_t(t);  // <= 63
+        const double sc = __shfl(scale_l, int(c), 64);
+        if (r < KB) zt[(4 * q + t) * KB + r] = c < f.count ? Real(sc * z[t]) : Real(0);
+    }
+    wave_lds_sync();
+    left_group<KB, FP32, WIDE>(stage, zt, lmeta_l, g0, lane, rows);
+    wave_lds_sync();  // the next group rewrites zt
+}
+
+// kRight components [g0, g0 + 16): right-factor sums into the workspace rows
+// of the block's column tile.
+template <int KB, bool FP32, bool WIDE>
+__device__ __forceinline__ void right_rows(const MemberView& M, const Item& f,
+                                           const std::uint32_t* stage, const double* xs,
+                                           std::uint32_t rmeta_l, std::uint32_t g0, int lane,
+                                           int p0, int kk) {
+    const int q = lane >> 4, r = lane & 15;
+    double z[4];
+    right_group<KB, FP32, WIDE>(stage, xs, rmeta_l, g0, lane, z);
+    if (r < kk) {
+        double* out = M.zp + (std::uint64_t(f.zrow) + g0 + 4u * std::uint32_t(q)) *
+                                 std::uint64_t(M.k) + p0 + r;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (g0 + 4u * std::uint32_t(q) + std::uint32_t(t) < f.count)
+                out[std::uint64_t(t) * std::uint64_t(M.k)] = z[t];
+    }
+}
+
+// The registers of an item that outlive its staging: component widths and
+// scale products (lanes = components).
+struct Staged {
+    std::uint32_t bits_l;
+    double scale_l;
+};
+
+// Moves a fetched item into the wave's LDS: its data, and its x rows (lanes =
+// columns; zero past the item's columns) or kLeft partial sums (lanes =
+// pairs; zero past them), and keeps the rest.
+template <int KB>
+__device__ __forceinline__ Staged stage_item(const Item& f, const Fetch<KB>& F, int lane,
+                                             std::uint8_t* wave_lds) {
+    quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds + kLdsHead);
+    double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
 #pragma unroll
     for (int t = 0; t < kItemQuads / 64; ++t)
         if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
-    Staged<KB> st;
-    st.bits_l = F.bits;
-    st.scale_l = F.scale;
 #pragma unroll
-    for (int p = 0; p < KB; ++p) st.in[p] = F.in[p];
-    if (f.kind == kDense || f.kind == kLeft) {
-        if (std::uint32_t(lane) < max(x_columns(f), z_pairs(f))) {
-#pragma unroll
-            for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
-        }
-    }
-    return st;
+    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+    return Staged{F.bits, F.scale};
 }
 
-// Decodes and applies one staged item.  Pass 1 holds kDense, kFused and
-// kRight items, pass 2 kLeft items only.
-template <int KB, bool FP32, int PASS>
-__device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
-                                             const Staged<KB>& st, int lane, int p0, int kk,
+// A kFused, kRight (pass 1) or kLeft (pass 2) item, 16 components at a
+// time.  Row contributions are formed on the matrix cores (in FP32 for the
+// codes in FP32 mode) and added to acc (lanes = rows) in FP64.
+template <int KB, bool FP32, int PASS, bool WIDE>
+__device__ __forceinline__ void lowrank_item(const MemberView& M, const Item& f,
+                                             const Staged& st, int lane, int p0, int kk,
                                              std::uint32_t R, std::uint8_t* wave_lds,
                                              double (&acc)[KB]) {
     using Real = real_t<FP32>;
-    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds);
-    const double* xs = reinterpret_cast<const double*>(wave_lds + kStageBytes);
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds + kLdsHead);
+    double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
+    Real* zt = reinterpret_cast<Real*>(wave_lds + kLdsHead + kStageBytes + kTile * KB * 8);
     const bool comp = std::uint32_t(lane) < f.count;
     const std::uint32_t bits_l = st.bits_l;
     const double scale_l = st.scale_l;
-    const double (&in)[KB] = st.in;
+    ItemRows<FP32> rows;
+    if (PASS == 2) {
+        // kLeft. z of each component: its column tiles' partial sums in order
+        double z[KB];
+#pragma unroll
+        for (int p = 0; p < KB; ++p) z[p] = 0.0;
+        const std::uint32_t nseg = f.nseg;
+        if (comp && (lane + 1) * nseg <= 64u) {
+            for (std::uint32_t sg = 0; sg < nseg; ++sg) {
+#pragma unroll
+                for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
+            }
+        }
+        const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
+        const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
+        for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16) {
+            if (std::uint32_t(lane) - g0 < 16u) {
+#pragma unroll
+                for (int p = 0; p < KB; ++p) zt[(std::uint32_t(lane) - g0) * KB + p] = Real(scale_l * z[p]);
+            }
+            wave_lds_sync();
+            left_group<KB, FP32, WIDE>(stage, zt, lmeta_l, g0, lane, rows);
+            wave_lds_sync();  // the next group rewrites zt
+        }
+    } else {
+        const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
+        const std::uint32_t incl_r = wave_scan(nwr, lane);
+        const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
+        if (f.kind == kRight) {
+            for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
+                right_rows<KB, FP32, WIDE>(M, f, stage, xs, rmeta_l, g0, lane, p0, kk);
+            return;
+        }
+        const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
+        const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
+        const std::uint32_t lmeta_l =
+            pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
+        for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
+            fused_group<KB, FP32, WIDE>(f, stage, xs, zt, rmeta_l, lmeta_l, scale_l, g0, lane, rows);
+    }
+    add_item_rows<KB, FP32>(rows, xs, lane, acc);
+}
+
+// Narrow passes: one staged item on the vector ALUs.  Row contributions are
+// added to acc (lanes = the R rows of the tile), in FP32 per item for the
+// codes in FP32 mode.
+template <int KB, bool FP32, int PASS>
+__device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, const Staged& st,
+                                          int lane, int p0, int kk, std::uint32_t R,
+                                          std::uint8_t* wave_lds, double (&acc)[KB]) {
+    using Real = real_t<FP32>;
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds + kLdsHead);
+    const double* xs = reinterpret_cast<const double*>(wave_lds + kLdsHead + kStageBytes);
+    const bool comp = std::uint32_t(lane) < f.count;
+    const std::uint32_t bits_l = st.bits_l;
+    const double scale_l = st.scale_l;
     float blk[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
@@ -582,7 +876,9 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
                 for (int p = 0; p < KB; ++p) acc[p] = fma(v, xs[c * KB + p], acc[p]);
             }
         }
-    } else if (PASS == 2) {
+        return;
+    }
+    if (PASS == 2) {
         // kLeft. z of each component: its column tiles' partial sums in order
         double z[KB];
 #pragma unroll
@@ -599,14 +895,14 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
         for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
         const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
         const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
-        left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
-                            [&](std::uint32_t j, int p) { return lane_value(zs_l[p], j); },
-                            acc, blk);
+        valu_left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
+                                 [&](std::uint32_t j, int u, int p) { return lane_value(zs_l[p], j + u); },
+                                 acc, blk);
     } else {
         // kFused / kRight: lanes = columns for the right factor
         Real xl[KB];
 #pragma unroll
-        for (int p = 0; p < KB; ++p) xl[p] = lane < int(f.cols) ? Real(in[p]) : Real(0);
+        for (int p = 0; p < KB; ++p) xl[p] = Real(xs[lane * KB + p]);  // zero past the columns
         const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
         const std::uint32_t incl_r = wave_scan(nwr, lane);
         const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
@@ -619,15 +915,39 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
         // split path)
         std::uint32_t j0 = 0;
         for (; j0 + 4 < f.count; j0 += 8)
-            right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0, kk,
-                                         xl, acc, blk);
+            valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0,
+                                              kk, xl, acc, blk);
         if (j0 < f.count)
-            right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0, kk,
-                                         xl, acc, blk);
+            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0,
+                                              kk, xl, acc, blk);
     }
-    if (FP32 && f.kind != kDense) {
+    if (FP32) {
 #pragma unroll
         for (int p = 0; p < KB; ++p) acc[p] += double(blk[p]);
+    }
+}
+
+// Decodes and applies one staged item.  Pass 1 holds kDense, kFused and
+// kRight items, pass 2 kLeft items only.  Narrow passes run on the vector
+// ALUs, wide ones on the matrix cores; in FP32 mode on the matrix cores an
+// item with a code wider than 24 bits takes the split path throughout.
+template <int KB, bool FP32, int PASS>
+__device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
+                                             const Staged& st, int lane, int p0, int kk,
+                                             std::uint32_t R, std::uint8_t* wave_lds,
+                                             double (&acc)[KB]) {
+    if constexpr (KB < kMatrixMinKB) {
+        valu_item<KB, FP32, PASS>(M, f, st, lane, p0, kk, R, wave_lds, acc);
+    } else if (PASS == 1 && f.kind == kDense) {
+        double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
+        ItemRows<false> rows;
+        dense_rows<KB>(reinterpret_cast<const double*>(wave_lds + kLdsHead), xs, R,
+                       min(f.count, std::uint32_t(kDenseColumns)), lane, rows);
+        add_item_rows<KB, false>(rows, xs, lane, acc);
+    } else if (FP32 && __ballot(std::uint32_t(lane) < f.count && st.bits_l > 24u) != 0) {
+        lowrank_item<KB, FP32, PASS, FP32>(M, f, st, lane, p0, kk, R, wave_lds, acc);
+    } else {
+        lowrank_item<KB, FP32, PASS, false>(M, f, st, lane, p0, kk, R, wave_lds, acc);
     }
 }
 
@@ -667,7 +987,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
         // steady state: buffer A holds item s, buffer B item s+1
         for (std::uint32_t s = 0; s < n; s += 2) {
             {
-                const Staged<KB> st = stage_item<KB>(fa, FA, lane, wave_lds);
+                const Staged st = stage_item<KB>(fa, FA, lane, wave_lds);
                 fetch<KB>(M, fc, pc, lane, p0, kk, FA);
                 const Item fd = slot(s + 3);
                 const std::int32_t pd = perm_row(M, fd, lane);
@@ -680,7 +1000,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
             }
             if (s + 1 >= n) break;
             {
-                const Staged<KB> st = stage_item<KB>(fb, FB, lane, wave_lds);
+                const Staged st = stage_item<KB>(fb, FB, lane, wave_lds);
                 fetch<KB>(M, fc, pc, lane, p0, kk, FB);
                 const Item fd = slot(s + 4);
                 const std::int32_t pd = perm_row(M, fd, lane);
@@ -836,7 +1156,9 @@ __device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t,
 
 // Pass-1 waves per SIMD the register allocation must allow, by right-hand
 // sides per pass.
-constexpr int pass1_occupancy(int kb) { return kb <= 1 ? 4 : kb <= 8 ? 2 : 1; }
+constexpr int pass1_occupancy(int kb) { return kb <= 1 ? 4 : 2; }
+// The batch kernels carry the member table besides.
+constexpr int batch1_occupancy(int kb) { return kb <= 1 ? 3 : 2; }
 
 template <int KB, bool FP32>
 __global__ __launch_bounds__(64 * kWaves / kPass1Groups, pass1_occupancy(KB)) void apply_pass1(
@@ -876,7 +1198,7 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
 }
 
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kWaves / kPass1Groups, pass1_occupancy(KB)) void batch_pass1(
+__global__ __launch_bounds__(64 * kWaves / kPass1Groups, batch1_occupancy(KB)) void batch_pass1(
     const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
     std::uint32_t count, const double* x, double* y, double* zp) {
     __shared__ __align__(16) std::uint8_t lds[kWaves / kPass1Groups * wave_lds_bytes<KB>()];
@@ -960,7 +1282,7 @@ int rhs_block(const DeviceStore& s, std::int64_t k) {
     return it == s.rhs_block.end() ? default_rhs_block(k) : it->second;
 }
 
-constexpr int waves_for(int kb) { return kb >= 16 ? 4 : kb >= 2 ? 8 : 16; }
+constexpr int waves_for(int kb) { return kb >= 2 ? 8 : 16; }
 
 template <int KB, bool FP32>
 void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,

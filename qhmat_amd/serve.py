@@ -13,8 +13,10 @@ batching of apply requests, p50/p99 latency.
   columns of the result.
 
 A right-hand-side column's arithmetic does not depend on which other columns
-share its pass, so a request's result is bitwise the same alone and in any
-mix (tests/test_gpu_serve.py).
+share its pass, only on the pass's arithmetic path (vector ALUs for narrow
+passes, matrix cores for 16-wide ones): a request's result is bitwise the same
+alone and in a mix that keeps it on its path, and within the precision's
+tolerance otherwise (tests/test_gpu_serve.py).
 """
 
 from __future__ import annotations

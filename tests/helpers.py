@@ -89,6 +89,24 @@ def rel_err(a, b) -> float:
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+# Right-hand sides per pass from which the device passes form the code
+# products on the matrix cores (narrower passes use the vector ALUs).  Within
+# one of the two paths a column's bits do not depend on the pass width; across
+# them results agree to the precision's tolerance (same_result).
+MATRIX_MIN_KB = 8
+FP32_REL_TOL = 1e-6
+
+
+def same_result(got, want, precision: str) -> None:
+    """The same apply through the other arithmetic path: FP64 within rtol =
+    atol = 1e-12 (scaled), FP32-split within relative error 1e-6."""
+    got, want = np.asarray(got), np.asarray(want)
+    if precision == "fp64":
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * float(np.abs(want).max()))
+    else:
+        assert rel_err(got, want) <= FP32_REL_TOL
+
+
 def wide_strips(tiles: int = 68, rank: int = 3, seed: int = 7):
     """Row strips of 64: a dense 64x64 diagonal block between two low-rank
     blocks spanning the rest of the strip.  The first and last strips hold

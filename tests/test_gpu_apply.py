@@ -114,13 +114,19 @@ def test_reference_cuda_case(stores):
 @pytest.mark.parametrize("precision", ["fp64", "fp32split"])
 def test_bitwise_reproducible_and_rhs_count_independent(stores, precision):
     native = stores["ragged_square"][0]
-    x = np.random.default_rng(2).normal(size=(native.order, 9))
-    a = native.matmul(x, True, precision=precision)
-    b = native.matmul(x, True, precision=precision)
+    x = np.random.default_rng(2).normal(size=(native.order, 20))
+    a = native.matmul(x[:, :9], True, precision=precision)  # one matrix-core pass
+    b = native.matmul(x[:, :9], True, precision=precision)
     np.testing.assert_array_equal(a, b)
-    # each column alone gives the same bits as inside the 9-wide pass
+    # the same columns inside two 16-wide passes: the same bits
+    np.testing.assert_array_equal(native.matmul(x, True, precision=precision)[:, :9], a)
+    # narrow passes (vector path): a column alone and inside a 3-wide pass
+    c = native.matmul(x[:, :3], True, precision=precision)
+    for p in range(3):
+        np.testing.assert_array_equal(native.matmul(x[:, p], True, precision=precision), c[:, p])
+    # across the two paths: within the precision's tolerance
     for p in (0, 4, 8):
-        np.testing.assert_array_equal(native.matmul(x[:, p], True, precision=precision), a[:, p])
+        H.same_result(native.matmul(x[:, p], True, precision=precision), a[:, p], precision)
 
 
 def test_device_tensors_strided_and_workspace(stores, torch):
