@@ -250,6 +250,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the served-load measurement (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=4000.0)
+    ap.add_argument("--harness", action="store_true",
+                    help="also time the application-harness methods (dense GEMM, batched GEMMs on "
+                         "decoded factors, the fused apply) on the same store at k = 1 and 32")
     args = ap.parse_args()
     if 1 not in args.rhs:
         args.rhs = [1] + args.rhs  # the headline value is quoted at k=1
@@ -317,6 +320,17 @@ def main() -> int:
         serving = measure_serving(torch, served, args.serve_rate, args.serve_seconds)
         log(f"serving: p50 {serving.get('latency_p50_ms')} ms, p99 {serving.get('latency_p99_ms')} ms "
             f"at {serving['throughput_rps']:.0f} req/s")
+
+    harness = None
+    if args.harness:
+        from qhmat_amd.harness import Harness
+
+        log("harness: reconstructing and decoding the store")
+        hn = Harness(native)
+        harness = {f"k{k}": hn.run(k) for k in (1, 32)}
+        for kk, res in harness.items():
+            log(f"harness {kk}: " + ", ".join(f"{m} {v['ms'] * 1e3:.1f} us" for m, v in res.items()))
+        del hn
 
     base = None
     y_cpu = None
@@ -390,6 +404,8 @@ def main() -> int:
         line["served"] = serving
         line["served_p50_ms"] = serving.get("latency_p50_ms")
         line["served_p99_ms"] = serving.get("latency_p99_ms")
+    if harness is not None:
+        line["harness"] = harness
     if base is not None:
         line["cpu_baseline"] = {
             "value": round(b1["total"] / base["seconds"] / 1e9, 4), "unit": "GB/s",

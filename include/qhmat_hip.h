@@ -179,6 +179,18 @@ qh_status qh_profile_end(qh_store *store, qh_profile *out);
  * test hook for the tile-major layout; no apply path calls it. */
 qh_status qh_plan_emulate(const qh_store_desc *desc, const double *x, int64_t k, double *y);
 
+/* Dense reconstruction on the device: writes the order x order FP64 matrix
+ * in source index order into device memory `out` (row-major, leading
+ * dimension ld >= order), every entry formed as the host reconstruction
+ * forms it, so the result is bitwise the host's.  Stream-ordered; the first
+ * call on a store uploads its reconstruction tables (per-tile tasks, chunk
+ * offsets, per-component scales).
+ * Replaces HMatrix::reconstruct (src/qhmat/core.cpp:392-410), bound as
+ * NativeHMatrix.reconstruct (src/qhmat/bindings.cpp:180-190) and
+ * HMatrix.to_dense (python/qhmat/matrix.py:309-311), for large-order checks
+ * without a host pass over n^2 entries. */
+qh_status qh_reconstruct(const qh_store *store, double *out, int64_t ld, void *stream);
+
 /* Test hook: the integer codes of one side (0 = left, 1 = right) of block
  * `block` (insertion index), decoded on the device from the resident words
  * into out[j*rows + i] (int32, device memory, rank*rows entries).  They must

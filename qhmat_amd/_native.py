@@ -107,6 +107,7 @@ _sig("qh_store_get_stats", _p, C.POINTER(StoreStats))
 _sig("qh_apply_workspace_bytes", _p, _i64, C.POINTER(_u64))
 _sig("qh_apply", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p, _p, _u64)
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
+_sig("qh_reconstruct", _p, _p, _i64, _p)
 _sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)
 _sig("qh_store_set_rhs_block", _p, _i64, C.c_int)
 _sig("qh_profile_begin", _p, _u64)
@@ -506,6 +507,14 @@ class NativeHMatrix:
 
         return _device.apply(self, x, y, precision=precision, stream=stream,
                              workspace=workspace)
+
+    def reconstruct_device(self, out=None, *, stream=None):
+        """Dense FP64 matrix in source index order, formed on the device
+        (qh_reconstruct): a torch tensor (order, order) on the current
+        device, or `out` (row-major, unit column stride) filled in place."""
+        from . import device as _device
+
+        return _device.reconstruct(self, out, stream=stream)
 
     def decode_codes(self, block: int, side: int):
         """Integer codes of one block side decoded on the device (test hook)."""

@@ -251,6 +251,13 @@ qh_status qh_profile_end(qh_store* store, qh_profile* out) {
     });
 }
 
+qh_status qh_reconstruct(const qh_store* store, double* out, int64_t ld, void* stream) {
+    return guarded([&] {
+        need(store, "store");
+        qhamd::reconstruct(*store->dev, out, ld, stream);
+    });
+}
+
 qh_status qh_decode_codes(const qh_store* store, uint64_t block, int side, int32_t* out,
                           void* stream) {
     return guarded([&] {

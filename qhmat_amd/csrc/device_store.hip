@@ -308,19 +308,17 @@ __device__ __forceinline__ double lane_value(double v, std::uint32_t j) {
     return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
-// Code `idx` of a stream whose first word is at `base` (LDS): the word pair
-// holding the code (v_alignbit_b32), then a shift pair that sign-extends the
-// b-bit field (also for b = 32).  No branches: lanes past the end of a
-// stream read the stage's slack words.
+// Code idx1 - 1 of a stream whose first word is at `base` (LDS, with the
+// word before it readable): the 32 bits that end with the code's last bit
+// (v_alignbit_b32 of the word pair), shifted arithmetically right by 32 - b,
+// which sign-extends the field (b = 32 included) and drops the bits below
+// it.  No branches: lanes past the end of a stream read the stage's slack.
 __device__ __forceinline__ int decode_lane(const std::uint32_t* base, std::uint32_t bits,
-                                           std::uint32_t idx) {
-    const std::uint32_t pos = __umul24(idx, bits);
-    const std::uint32_t* w = base + (pos >> 5);
-    const std::uint32_t raw = __builtin_amdgcn_alignbit(w[1], w[0], pos);
-    const std::uint32_t sh = 32u - bits;
-    return static_cast<int>(raw << sh) >> sh;
+                                           std::uint32_t idx1) {
+    const std::uint32_t end = __umul24(idx1, bits);
+    const std::uint32_t* w = base + __builtin_amdgcn_ubfe(end, 5u, 27u);  // end / 32
+    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-1], end)) >> (32u - bits);
 }
-
 
 // Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
 // lane-half swaps (v_permlane32_swap / v_permlane16_swap).
@@ -443,7 +441,7 @@ __device__ __forceinline__ void valu_left_four(const std::uint32_t* stage, const
                                           double (&acc)[KB], float (&blk)[KB]) {
     int code[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) code[u] = decode_lane(stage + (meta[u] & 0xFFFFu), meta[u] >> 16, lane);
+    for (int u = 0; u < 4; ++u) code[u] = decode_lane(stage + (meta[u] & 0xFFFFu), meta[u] >> 16, lane + 1);
     if (!FP32) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -475,21 +473,22 @@ __device__ __forceinline__ void valu_left_four(const std::uint32_t* stage, const
 // Left-factor rows of components [j0, jend) of an item, four at a time
 // (j0 a multiple of 4 and jend <= 64, so every lane index stays below 64):
 // meta held one per lane (lanes = components), zs_of(j, u, p) the scaled input
-// of component j for right-hand side p (lane-uniform), zero for components
-// past jend, whose dummy meta decodes in-bounds 1-bit codes.
+// of component j + u for right-hand side p (lane-uniform), zero for
+// components past jend, whose dummy meta decodes in-bounds 1-bit codes;
+// bit c of wide_mask set for a component c wider than 24 bits (FP32 mode).
 template <int KB, bool FP32, class ZsOf>
 __device__ __forceinline__ void valu_left_rows(const std::uint32_t* stage, std::uint32_t lmeta_l,
-                                          std::uint32_t j0, std::uint32_t jend, int lane,
-                                          ZsOf zs_of, double (&acc)[KB], float (&blk)[KB]) {
+                                          std::uint64_t wide_mask, std::uint32_t j0,
+                                          std::uint32_t jend, int lane, ZsOf zs_of,
+                                          double (&acc)[KB], float (&blk)[KB]) {
     using Real = real_t<FP32>;
     for (std::uint32_t j = j0; j < jend; j += 4) {
         std::uint32_t meta[4];
         Real zs[4][KB];
-        bool wide = false;
+        const bool wide = ((wide_mask >> j) & 0xFull) != 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             meta[u] = __builtin_amdgcn_readlane(lmeta_l, j + u);
-            wide = wide || (meta[u] >> 16) > 24u;
 #pragma unroll
             for (int p = 0; p < KB; ++p) zs[u][p] = zs_of(j, u, p);
         }
@@ -508,16 +507,15 @@ __device__ __forceinline__ void valu_left_rows(const std::uint32_t* stage, std::
 // rounding is the same for every right-hand-side count.
 template <int N, int KB, bool FP32>
 __device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std::uint32_t rmeta_l,
-                                            std::uint32_t j0, std::uint32_t count, int lane,
+                                            std::uint64_t wide_mask, std::uint32_t j0, int lane,
                                             const real_t<FP32> (&xl)[KB], double (&tot)[KB]) {
     using Real = real_t<FP32>;
     int code[N];
-    bool wide = false;
+    const bool wide = ((wide_mask >> j0) & ((1ull << N) - 1ull)) != 0;
 #pragma unroll
     for (int g = 0; g < N; ++g) {
         const std::uint32_t meta = __builtin_amdgcn_readlane(rmeta_l, min(j0 + g, 63u));
-        wide = wide || (meta >> 16) > 24u;
-        code[g] = decode_lane(stage + (meta & 0xFFFFu), meta >> 16, lane);
+        code[g] = decode_lane(stage + (meta & 0xFFFFu), meta >> 16, lane + 1);
     }
     if (!FP32 || !wide) {
 #pragma unroll
@@ -550,13 +548,13 @@ __device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std
 template <int N, int KB, bool FP32>
 __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const Item& f,
                                                 const std::uint32_t* stage, std::uint32_t rmeta_l,
-                                                std::uint32_t lmeta_l, double scale_l,
-                                                std::uint32_t j0, int lane, int p0, int kk,
-                                                const real_t<FP32> (&xl)[KB], double (&acc)[KB],
-                                                float (&blk)[KB]) {
+                                                std::uint32_t lmeta_l, std::uint64_t wide_mask,
+                                                double scale_l, std::uint32_t j0, int lane, int p0,
+                                                int kk, const real_t<FP32> (&xl)[KB],
+                                                double (&acc)[KB], float (&blk)[KB]) {
     using Real = real_t<FP32>;
     double tot[KB];
-    valu_right_block<N, KB, FP32>(stage, rmeta_l, j0, f.count, lane, xl, tot);
+    valu_right_block<N, KB, FP32>(stage, rmeta_l, wide_mask, j0, lane, xl, tot);
     const std::uint32_t jl = j0 + std::uint32_t(block_slot<N>(lane));
     if (f.kind == kRight) {
         if (block_writer<N>(lane) && jl < f.count) {
@@ -571,7 +569,7 @@ __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const 
     Real zs_lane[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zs_lane[p] = jl < f.count ? Real(sc * tot[p]) : Real(0);
-    valu_left_rows<KB, FP32>(stage, lmeta_l, j0, min(j0 + N, f.count), lane,
+    valu_left_rows<KB, FP32>(stage, lmeta_l, wide_mask, j0, min(j0 + N, f.count), lane,
                         [&](std::uint32_t j, int u, int p) {
                             return lane_value(zs_lane[p], block_lane<N>(int(j + u - j0)));
                         },
@@ -861,6 +859,8 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
     const bool comp = std::uint32_t(lane) < f.count;
     const std::uint32_t bits_l = st.bits_l;
     const double scale_l = st.scale_l;
+    // components wider than 24 bits (split in FP32 mode), one bit each
+    const std::uint64_t wide_mask = FP32 ? __ballot(comp && bits_l > 24u) : 0ull;
     float blk[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) blk[p] = 0.0f;
@@ -895,7 +895,7 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
         for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
         const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
         const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
-        valu_left_rows<KB, FP32>(stage, lmeta_l, 0, f.count, lane,
+        valu_left_rows<KB, FP32>(stage, lmeta_l, wide_mask, 0, f.count, lane,
                                  [&](std::uint32_t j, int u, int p) { return lane_value(zs_l[p], j + u); },
                                  acc, blk);
     } else {
@@ -915,11 +915,11 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
         // split path)
         std::uint32_t j0 = 0;
         for (; j0 + 4 < f.count; j0 += 8)
-            valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0,
-                                              kk, xl, acc, blk);
+            valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, wide_mask, scale_l, j0,
+                                              lane, p0, kk, xl, acc, blk);
         if (j0 < f.count)
-            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, scale_l, j0, lane, p0,
-                                              kk, xl, acc, blk);
+            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, wide_mask, scale_l, j0,
+                                              lane, p0, kk, xl, acc, blk);
     }
     if (FP32) {
 #pragma unroll
@@ -1225,6 +1225,96 @@ __global__ __launch_bounds__(64 * W) void batch_pass2(const MemberView* __restri
 
 // Test hook: codes of one factor side of one block, from its chunks, written
 // component-major in the block's own row order.
+// Reconstruction, one workgroup per (block, row tile, column tile): every
+// entry formed as the reference forms it (core.cpp:392-410), in FP64 without
+// contraction: the diagonal block's value, or zero plus, component by
+// component, (code_l lscale) (code_r rscale).  The factors of 32 components
+// at a time are decoded into LDS; thread 4i + q holds row i and columns
+// q, q + 4, ... of the tile and writes them through the permutation.
+constexpr int kRecComps = 32;
+__global__ __launch_bounds__(256) void reconstruct_tiles(
+    const RecTask* __restrict__ tasks, const ChunkRef* __restrict__ chunks,
+    const std::uint32_t* __restrict__ words, const std::int32_t* __restrict__ widths,
+    const double* __restrict__ lscale, const double* __restrict__ rscale,
+    const std::int32_t* __restrict__ perm, double* __restrict__ out, std::int64_t ld) {
+    __shared__ double L[kTile][kRecComps + 1], Rt[kTile][kRecComps + 1];
+    __shared__ std::uint64_t off[2][kRecComps];
+    const RecTask t = tasks[blockIdx.x];
+    const int tid = int(threadIdx.x);
+    const int i = tid >> 2, kq = tid & 3;
+    const int rows = int(min(t.rows, std::uint32_t(kTile))), cols = int(min(t.cols, std::uint32_t(kTile)));
+    double v[kTile / 4];
+#pragma unroll
+    for (int e = 0; e < kTile / 4; ++e) v[e] = 0.0;
+    if (t.kind == 1) {
+        // the block's row tile: items of 8 columns, each R x 8 column-major
+        const double* dv = reinterpret_cast<const double*>(words + 4ull * t.quad);
+#pragma unroll
+        for (int e = 0; e < kTile / 4; ++e) {
+            const int k = kq + 4 * e;
+            if (i < rows && k < cols) {
+                const std::uint32_t cb = t.col0 + std::uint32_t(k) - t.bcol0;
+                v[e] = dv[std::uint64_t(cb / kDenseColumns) * kDenseColumns * std::uint64_t(rows) +
+                          (cb % kDenseColumns) * std::uint64_t(rows) + std::uint64_t(i)];
+            }
+        }
+    } else if (t.kind == 2) {
+        for (std::uint32_t w0 = 0; w0 < t.rank; w0 += kRecComps) {
+            const std::uint32_t wn = min(std::uint32_t(kRecComps), t.rank - w0);
+            __syncthreads();
+            if (tid < 2 * kRecComps) {
+                // first word of component w0 + jj's stream in this tile's chunk
+                const int side = tid / kRecComps, jj = tid % kRecComps;
+                const std::uint32_t j = w0 + std::uint32_t(jj);
+                const ChunkRef* c = chunks + (side ? t.rchunk : t.lchunk);
+                const std::uint32_t n = side ? t.nr : t.nl;
+                std::uint64_t w = 0;
+                for (std::uint32_t q = 0; q < n; ++q) {
+                    if (j >= c[q].j0 && j < c[q].j0 + c[q].jn) {
+                        w = c[q].word;
+                        for (std::uint32_t jp = c[q].j0; jp < j; ++jp)
+                            w += stream_words(c[q].count, std::uint32_t(widths[t.comp_base + jp]));
+                    }
+                }
+                off[side][jj] = w;
+            }
+            __syncthreads();
+            for (int p = tid; p < kTile * kRecComps; p += 256) {
+                const int row = p / kRecComps, jj = p % kRecComps;
+                if (std::uint32_t(jj) < wn) {
+                    const std::uint32_t c = t.comp_base + w0 + std::uint32_t(jj);
+                    const std::uint32_t bits = std::uint32_t(widths[c]);
+                    if (row < rows)
+                        L[row][jj] = __dmul_rn(double(decode_code(words + off[0][jj], std::uint32_t(row), bits)),
+                                               lscale[c]);
+                    if (row < cols)
+                        Rt[row][jj] = __dmul_rn(double(decode_code(words + off[1][jj], std::uint32_t(row), bits)),
+                                                rscale[c]);
+                }
+            }
+            __syncthreads();
+            if (i < rows) {
+                for (std::uint32_t jj = 0; jj < wn; ++jj) {
+                    const double l = L[i][jj];
+#pragma unroll
+                    for (int e = 0; e < kTile / 4; ++e) {
+                        const int k = kq + 4 * e;
+                        if (k < cols) v[e] = __dadd_rn(v[e], __dmul_rn(l, Rt[k][jj]));
+                    }
+                }
+            }
+        }
+    }
+    if (i < rows) {
+        const std::int64_t orow = std::int64_t(perm[t.row0 + std::uint32_t(i)]) * ld;
+#pragma unroll
+        for (int e = 0; e < kTile / 4; ++e) {
+            const int k = kq + 4 * e;
+            if (k < cols) out[orow + perm[t.col0 + std::uint32_t(k)]] = v[e];
+        }
+    }
+}
+
 __global__ void decode_side(const std::uint32_t* __restrict__ words,
                             const ChunkRef* __restrict__ chunks,
                             const std::int32_t* __restrict__ widths, std::uint32_t rows,
@@ -1456,6 +1546,7 @@ Plan build_plan(const qh_store_desc& d) {
     std::uint64_t codes = 0, lr_flops = 0, dn_flops = 0, z_rows = 0, dense_values = 0;
     std::uint64_t n_lowrank_all = 0, left_words = 0, right_words = 0;
     std::uint64_t pass_words[2] = {0, 0}, fused_components = 0;
+    std::vector<std::vector<std::uint32_t>> dense_quad(d.nblocks);  // per dense block, row tile
 
     // Starts an item's data on a 16-byte boundary; returns its first quad.
     auto begin_item = [&words]() {
@@ -1486,6 +1577,7 @@ Plan build_plan(const qh_store_desc& d) {
                     const std::uint64_t cnt = std::min<std::uint64_t>(kDenseColumns, nc - c);
                     ItemDev it{};
                     it.quad = begin_item();
+                    if (c == 0) dense_quad[b].push_back(it.quad);
                     it.kind = kDense;
                     it.count = std::uint8_t(cnt);
                     it.col = std::uint32_t(c0 + c);
@@ -1520,6 +1612,8 @@ Plan build_plan(const qh_store_desc& d) {
             require(bits >= 2 && bits <= 32, "bit width must be in [2, 32]");
             plan.widths.push_back(bits);
             plan.cscale.push_back(d.lscale[b][j] * d.rscale[b][j]);
+            plan.rec.lscale.push_back(d.lscale[b][j]);
+            plan.rec.rscale.push_back(d.rscale[b][j]);
             loff[j] = lw;
             roff[j] = rw;
             lcnt[j] = (m * bits + 31) / 32;
@@ -1649,6 +1743,42 @@ Plan build_plan(const qh_store_desc& d) {
     plan.stored_words = words.size();
     words.resize(words.size() + 8, 0u);  // one readable word past the last item
 
+    // Reconstruction: one task per (block, row tile, column tile).
+    for (std::uint64_t b = 0; b < d.nblocks; ++b) {
+        const std::uint64_t r0 = d.rect[4 * b], r1 = d.rect[4 * b + 1];
+        const std::uint64_t c0 = d.rect[4 * b + 2], c1 = d.rect[4 * b + 3];
+        const std::size_t rt0 = first_tile(row_tiles, r0), rt1 = first_tile(row_tiles, r1);
+        const std::size_t ct0 = first_tile(col_tiles, c0), ct1 = first_tile(col_tiles, c1);
+        const std::int64_t li = plan.block_to_lowrank[b];
+        for (std::size_t t = rt0; t < rt1; ++t)
+            for (std::size_t u = ct0; u < ct1; ++u) {
+                RecTask k{};
+                k.row0 = std::uint32_t(row_tiles[t].first);
+                k.rows = std::uint32_t(row_tiles[t].second - row_tiles[t].first);
+                k.col0 = std::uint32_t(col_tiles[u].first);
+                k.cols = std::uint32_t(col_tiles[u].second - col_tiles[u].first);
+                if (d.is_dense[b]) {
+                    k.kind = 1;
+                    k.quad = dense_quad[b].at(t - rt0);
+                    k.bcol0 = std::uint32_t(c0);
+                } else if (li >= 0) {
+                    k.kind = 2;
+                    k.rank = plan.ranks[li];
+                    k.comp_base = plan.comp_base[li];
+                    k.lchunk = std::uint32_t(plan.rec.chunks.size());
+                    // chunk rows and columns count from the block's first
+                    for (const auto& ch : plan.left_chunks[li])
+                        if (ch.first == k.row0 - r0) plan.rec.chunks.push_back(ch);
+                    k.nl = std::uint32_t(plan.rec.chunks.size()) - k.lchunk;
+                    k.rchunk = std::uint32_t(plan.rec.chunks.size());
+                    for (const auto& ch : plan.right_chunks[li])
+                        if (ch.first == k.col0 - c0) plan.rec.chunks.push_back(ch);
+                    k.nr = std::uint32_t(plan.rec.chunks.size()) - k.rchunk;
+                }
+                plan.rec.tasks.push_back(k);
+            }
+    }
+
     // Item table: right-factor items, then per tile its pass-1 and pass-2 items.
     std::vector<ItemDev>& items = plan.items;
     items = right_items;
@@ -1725,6 +1855,28 @@ Plan build_plan(const qh_store_desc& d) {
     return plan;
 }
 
+void reconstruct(const DeviceStore& s, double* out, std::int64_t ld, void* stream) {
+    require(out != nullptr, "null output pointer");
+    require(ld >= std::int64_t(s.order), "leading dimension is smaller than the matrix order");
+    DeviceScope scope(s.device);
+    {
+        std::lock_guard<std::mutex> guard(s.rec.lock);
+        if (!s.rec.uploaded) {
+            std::uint64_t bytes = 0;
+            s.rec.tasks = upload(s.rec.host.tasks, bytes);
+            s.rec.chunks = upload(s.rec.host.chunks, bytes);
+            s.rec.lscale = upload(s.rec.host.lscale, bytes);
+            s.rec.rscale = upload(s.rec.host.rscale, bytes);
+            s.rec.uploaded = true;
+        }
+    }
+    if (s.rec.host.tasks.empty()) return;
+    hipLaunchKernelGGL(reconstruct_tiles, dim3(unsigned(s.rec.host.tasks.size())), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), s.rec.tasks, s.rec.chunks, s.words,
+                       s.widths, s.rec.lscale, s.rec.rscale, s.perm, out, ld);
+    hip_check(hipGetLastError(), "reconstruct launch");
+}
+
 DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     const int count = device_count();
     if (count == 0) throw std::runtime_error("no CUDA device is available (no HIP device found)");
@@ -1758,6 +1910,7 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     s->comp_base = std::move(plan.comp_base);
     s->ranks = std::move(plan.ranks);
     s->stats = plan.stats;
+    s->rec.host = std::move(plan.rec);
     s->stats.device_bytes = s->device_bytes;
     s->stats.device = device;
     return s;
@@ -1910,7 +2063,9 @@ void destroy_device_store(DeviceStore* s) {
     for (void* e : s->profile.events) (void)hipEventDestroy(static_cast<hipEvent_t>(e));
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
                     static_cast<void*>(s->widths), static_cast<void*>(s->cscale),
-                    static_cast<void*>(s->items), static_cast<void*>(s->tiles)})
+                    static_cast<void*>(s->items), static_cast<void*>(s->tiles),
+                    static_cast<void*>(s->rec.tasks), static_cast<void*>(s->rec.chunks),
+                    static_cast<void*>(s->rec.lscale), static_cast<void*>(s->rec.rscale)})
         if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete s;

@@ -37,6 +37,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/qhmat_hip.h"
@@ -110,6 +111,25 @@ struct ChunkRef {
     std::uint64_t word;
 };
 
+// One (block, row tile, column tile) of a reconstruction (qh_reconstruct).
+struct RecTask {
+    std::uint32_t row0, rows, col0, cols;  // clustered rows and columns of the tile
+    std::uint32_t kind;                    // 0: zero (rank 0), 1: dense, 2: low rank
+    std::uint32_t rank, comp_base;         // low rank: components
+    std::uint32_t lchunk, nl, rchunk, nr;  // low rank: runs of the two sides' chunks
+    std::uint32_t quad;                    // dense: first item of the block's row tile
+    std::uint32_t bcol0;                   // dense: first column of the block
+    std::uint32_t pad[3];
+};
+static_assert(sizeof(RecTask) == 64, "RecTask is 64 bytes");
+
+// Tables of a reconstruction, built with the plan and uploaded on first use.
+struct RecTables {
+    std::vector<RecTask> tasks;
+    std::vector<ChunkRef> chunks;
+    std::vector<double> lscale, rscale;  // per component, as the factors were packed
+};
+
 struct DeviceStore {
     int device = 0;
     std::uint64_t order = 0;
@@ -134,6 +154,18 @@ struct DeviceStore {
     // Right-hand sides per pass chosen for a right-hand-side count
     // (qh_store_set_rhs_block, e.g. by the autotuner); others use the default.
     std::map<std::int64_t, int> rhs_block;
+    // Reconstruction tables: host copies, device copies made by the first
+    // qh_reconstruct.
+    struct Rec {
+        RecTables host;
+        RecTask* tasks = nullptr;
+        ChunkRef* chunks = nullptr;
+        double* lscale = nullptr;
+        double* rscale = nullptr;
+        bool uploaded = false;
+        std::mutex lock;
+    };
+    mutable Rec rec;
     // Optional per-pass event timing (qh_profile_begin / qh_profile_end).
     struct Profile {
         std::vector<void*> events;  // hipEvent_t, 3 per apply
@@ -157,6 +189,7 @@ struct Plan {
     std::vector<std::uint32_t> comp_base, ranks;
     std::uint64_t z_rows = 0, stored_words = 0;
     qh_store_stats stats{};
+    RecTables rec;
 };
 
 // Throw std::invalid_argument / std::runtime_error like the host store.
@@ -208,6 +241,9 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
            std::uint64_t workspace_bytes);
 void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int32_t* out,
                   void* stream);
+// Dense n x n FP64 matrix in source order into device memory (row-major,
+// leading dimension ld), stream-ordered.
+void reconstruct(const DeviceStore& s, double* out, std::int64_t ld, void* stream);
 int device_count();
 void profile_begin(DeviceStore& s, std::uint64_t max_applies);
 void set_rhs_block(DeviceStore& s, std::int64_t k, int kb);

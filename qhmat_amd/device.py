@@ -120,6 +120,25 @@ def apply_numpy(native, a: np.ndarray, precision: str = "fp64") -> np.ndarray:
     return y.cpu().numpy()
 
 
+def reconstruct(native, out=None, *, stream=None):
+    """Dense reconstruction on the device (qh_reconstruct); see
+    NativeHMatrix.reconstruct_device."""
+    require_device()
+    torch = _torch()
+    n = native.order
+    if out is None:
+        out = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    elif (out.dtype != torch.float64 or out.device.type != "cuda" or out.dim() != 2
+          or tuple(out.shape) != (n, n) or out.stride(1) != 1):
+        raise ValueError("out must be a float64 device tensor of shape (n, n) with unit column stride")
+    store = native.device_store(out.device.index if out.device.index is not None else 0)
+    st = stream if stream is not None else torch.cuda.current_stream(out.device)
+    handle = C.c_void_p(st.cuda_stream if hasattr(st, "cuda_stream") else st)
+    _native.check(_native.lib().qh_reconstruct(store, C.c_void_p(out.data_ptr()), out.stride(0),
+                                               handle))
+    return out
+
+
 def decode_codes(native, block: int, side: int) -> np.ndarray:
     """Codes of one block side, (rank, rows) int32, decoded by the device."""
     require_device()

@@ -291,8 +291,16 @@ class HMatrix:
         """Apply on the host with the ``@`` operator."""
         return self.matmul(x)
 
-    def to_dense(self) -> np.ndarray:
-        """Reconstruct a dense FP64 array; allocates ``8*n*n`` output bytes."""
+    def to_dense(self, *, device: str = "cpu") -> np.ndarray:
+        """Reconstruct a dense FP64 array; allocates ``8*n*n`` output bytes.
+
+        ``device="cuda"`` forms it on the device from the resident store
+        (bitwise the host result) and copies it back (matrix.py:309-311
+        computes it on the host only)."""
+        if device not in ("cpu", "cuda"):
+            raise ValueError("device must be 'cpu' or 'cuda'")
+        if device == "cuda":
+            return self._native.reconstruct_device().cpu().numpy()
         return self._native.reconstruct()
 
     def save(self, path: str | os.PathLike[str]) -> None:
