@@ -361,7 +361,11 @@ class NativeHMatrix:
 
     @property
     def order(self) -> int:
-        return int(self._info().order)
+        # fixed at creation; cached because the info query walks every block
+        n = self.__dict__.get("_order")
+        if n is None:
+            n = self._order = int(self._info().order)
+        return n
 
     @property
     def logical_bits(self) -> int:

@@ -22,9 +22,11 @@ relative error against the FP64 fused apply.
 from __future__ import annotations
 
 import collections
+import ctypes as C
 
 import numpy as np
 
+from . import _native
 from . import device as _device
 
 
@@ -93,14 +95,25 @@ class Harness:
         ws = torch.empty(max(_device.workspace_bytes(self.native, k), 8), dtype=torch.uint8,
                          device="cuda")
         y64 = torch.empty_like(x64)
+        # the fused apply through the C ABI with its arguments built once, as a
+        # host code calling it in a loop would
+        lib = _native.lib()
+        store = self.native.device_store(torch.cuda.current_device())
+        args = (C.c_void_p(x64.data_ptr()), x64.stride(0), k, C.c_void_p(y64.data_ptr()),
+                y64.stride(0))
+        tail = (C.c_void_p(torch.cuda.current_stream().cuda_stream), C.c_void_p(ws.data_ptr()),
+                ws.numel())
+
+        def fused(precision):
+            _native.check(lib.qh_apply(store, *args, precision, *tail))
+            return y64
+
         methods = {
             "dense_fp64": lambda: torch.mm(self.dense64, x64),
             "dense_fp32": lambda: torch.mm(self.dense32, x32),
             "predecoded_batched_fp32": lambda: self.batched(x32),
-            "packed_fused_fp32split": lambda: _device.apply(self.native, x64, y64,
-                                                            precision="fp32split", workspace=ws),
-            "packed_fused_fp64": lambda: _device.apply(self.native, x64, y64, precision="fp64",
-                                                       workspace=ws),
+            "packed_fused_fp32split": lambda: fused(_native.PRECISION_FP32SPLIT),
+            "packed_fused_fp64": lambda: fused(_native.PRECISION_FP64),
         }
         ref = _device.apply(self.native, x64, precision="fp64").clone()
         out = {}
