@@ -369,7 +369,7 @@ def main() -> int:
     dom_ms = r1["passes_ms"][dominant]
     dom_bytes = b1[f"pass{dominant}"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    dom_kernel = ["apply_pass1<1, true>", "apply_pass2<1, true, 16>"][dominant]
+    dom_kernel = ["apply_pass1<1, true>", "apply_pass2<1, true>"][dominant]
     line = {
         "metric": "H-matvec GB/s (% HBM peak) FP32 vs FP64 at rel err; served p50/p99 ms",
         "value": round(head["gbps"], 2),

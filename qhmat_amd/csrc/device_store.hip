@@ -1,7 +1,7 @@
 // MI355X (gfx950) apply of a qhmat H-matrix, y = H x.
 //
 // The packed store is uploaded once as a list of work items (layout in
-// device_store.hpp).  An apply is two launches on the caller's stream, both
+// device_store.hpp).  An apply is three launches on the caller's stream, all
 // deterministic (fixed summation order, one writer per output element, no
 // floating-point atomics):
 //
@@ -11,9 +11,10 @@
 //          in a fixed tree per workgroup into one of four row-sum sets of
 //          the workspace, and a share of the kRight items: partial sums
 //          z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c] into the workspace.
-//  pass 2  one workgroup per row tile: the kLeft items (U z with z summed
-//          over the block's column tiles), added to the four pass-1 row sums
-//          and written once to y[perm[row]].
+//  pass 2  the same shape over the tile's kLeft items (U z with z summed
+//          over the block's column tiles), into four more row-sum sets.
+//  combine one small workgroup per row tile: the eight sets in a fixed tree,
+//          written once to y[perm[row]].
 //
 // Every wave walks a contiguous list of items with the data of the next two
 // items (<= 4 KiB each, four 16-byte buffer loads per lane), their component
@@ -1015,36 +1016,6 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
     }
 }
 
-// The row sums of a tile are a balanced binary tree over its kWaves runs of
-// items, each run summed in item order from zero:
-//   ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) + ...
-// A W-wave workgroup gives each wave kWaves/W consecutive runs and forms the
-// lower levels of the tree in registers, the rest in wave 0; the result is
-// the same bits for every W (so for every right-hand-side count).
-template <int KB, bool FP32, int W>
-__device__ __forceinline__ void wave_rows(const MemberView& M, const std::uint32_t* rsplit,
-                                          const std::uint32_t* split, int wave, int p0, int kk,
-                                          std::uint32_t R, std::uint8_t* wave_lds,
-                                          double (&total)[KB]) {
-    constexpr int step = kWaves / W;
-    double pair[KB];
-#pragma unroll
-    for (int i = 0; i < step; ++i) {
-        double acc[KB];
-#pragma unroll
-        for (int p = 0; p < KB; ++p) acc[p] = 0.0;
-        const int r = wave * step + i;
-        run_items<KB, FP32, 2>(M, 0u, 0u, split[r], split[r + 1], p0, kk, R, wave_lds, acc);
-#pragma unroll
-        for (int p = 0; p < KB; ++p) {
-            if (i == 0) total[p] = acc[p];
-            else if (i == 1) total[p] += acc[p];
-            else if (i == 2) { pair[p] = total[p]; total[p] = acc[p]; }
-            else total[p] = pair[p] + (total[p] + acc[p]);
-        }
-    }
-}
-
 // Completes the tree over the W wave totals in wave 0 (two right-hand sides
 // per round through LDS), adds it to `base` and hands the result to `store`.
 template <int KB, int W, class Store>
@@ -1081,14 +1052,20 @@ __device__ __forceinline__ void combine_rows(const double (&total)[KB], const do
     }
 }
 
-// Pass 1: workgroup q (of kPass1Groups) of tile t runs the tile's runs
-// 4q .. 4q+3, one per wave, and writes their tree sum to its own set of
-// rows in the workspace; pass 2 adds the sets as ((g0 + g1) + (g2 + g3)),
-// which completes the same balanced tree over the 16 runs.
-template <int KB, bool FP32>
-__device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t group, int p0,
-                                            std::uint8_t* lds) {
-    constexpr int W = kWaves / kPass1Groups;
+// The row sums of a tile: each pass's 16 item runs (each summed in item
+// order from zero) go to four workgroups of four waves; workgroup q of pass
+// P sums its waves as ((r0 + r1) + (r2 + r3)) into its own set of rows in the
+// workspace (set 4(P-1) + q), and the combine launch forms
+//   y = ((g0 + g1) + (g2 + g3)) + ((h0 + h1) + (h2 + h3))
+// over the pass-1 sets g and the pass-2 sets h: one fixed tree over the 32
+// runs, the same for every right-hand-side count and launch shape.
+constexpr int kGroupWaves = kWaves / kPass1Groups;
+constexpr int kRowSets = 2 * kPass1Groups;
+
+template <int KB, bool FP32, int PASS>
+__device__ __forceinline__ void group_rows(const MemberView& M, std::uint32_t group, int p0,
+                                           std::uint8_t* lds) {
+    constexpr int W = kGroupWaves;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int kk = min(KB, M.k - p0);
@@ -1099,12 +1076,18 @@ __device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t g
     double total[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) total[p] = 0.0;
-    run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
-                        tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(), total);
+    if (PASS == 1)
+        run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
+                               tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(),
+                               total);
+    else
+        run_items<KB, FP32, 2>(M, 0u, 0u, tile->split2[r], tile->split2[r + 1], p0, kk, R,
+                               lds + wave * wave_lds_bytes<KB>(), total);
     double zero[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zero[p] = 0.0;
-    double* rows = M.zp + (M.z_rows + q * M.order + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+    const std::uint32_t set = (PASS == 1 ? 0u : std::uint32_t(kPass1Groups)) + q;
+    double* rows = M.zp + (M.z_rows + set * M.order + tile->row0 + lane) * std::uint64_t(M.k) + p0;
     combine_rows<KB, W>(total, zero, lds, wave, lane, [&](int pc, double a0, double a1) {
         if (lane < int(R)) {
             if (pc < kk) rows[pc] = a0;
@@ -1113,45 +1096,21 @@ __device__ __forceinline__ void pass1_group(const MemberView& M, std::uint32_t g
     });
 }
 
-template <int KB, bool FP32, int W>
-__device__ __forceinline__ void pass2_tile(const MemberView& M, std::uint32_t t, int p0,
-                                           std::uint8_t* lds) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+// y rows of tile t: the tree over the eight row sets (lanes = rows).
+__device__ __forceinline__ void combine_tile(const MemberView& M, std::uint32_t t) {
     const int lane = threadIdx.x & 63;
-    const int kk = min(KB, M.k - p0);
     const TileDev* __restrict__ tile = M.tiles + t;
     const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
-    const bool active = lane < int(R);
-    // pass-1 rows of this tile, one set per pass-1 workgroup: requested first
-    // for up to two right-hand sides, at the end (fewer live registers) beyond
-    const std::uint64_t stride = std::uint64_t(M.order) * std::uint64_t(M.k);
-    const double* rows = M.zp + (M.z_rows + tile->row0 + lane) * std::uint64_t(M.k) + p0;
-    auto pass1_rows = [&](int p) {
-        double g[kPass1Groups];
+    if (std::uint32_t(lane) >= R) return;
+    const std::uint64_t k = std::uint64_t(M.k), stride = std::uint64_t(M.order) * k;
+    const double* rows = M.zp + (M.z_rows + tile->row0 + lane) * k;
+    double* y = M.y + std::int64_t(M.perm[tile->row0 + lane]) * M.ldy;
+    for (std::uint64_t p = 0; p < k; ++p) {
+        double v[kRowSets];
 #pragma unroll
-        for (int q = 0; q < kPass1Groups; ++q)
-            g[q] = (wave == 0 && active && p < kk) ? rows[q * stride + p] : 0.0;
-        return (g[0] + g[1]) + (g[2] + g[3]);
-    };
-    constexpr bool early = KB <= 2;
-    double base[KB];
-#pragma unroll
-    for (int p = 0; p < KB; ++p) base[p] = early ? pass1_rows(p) : 0.0;
-    const std::int64_t yrow = active ? std::int64_t(M.perm[tile->row0 + lane]) * M.ldy + p0 : 0;
-    double total[KB];
-    wave_rows<KB, FP32, W>(M, nullptr, tile->split2, wave, p0, kk, R,
-                           lds + wave * wave_lds_bytes<KB>(), total);
-    double* y = M.y;
-    combine_rows<KB, W>(total, base, lds, wave, lane, [&](int pc, double a0, double a1) {
-        if (!early) {
-            a0 += pass1_rows(pc);
-            if (pc + 1 < KB) a1 += pass1_rows(pc + 1);
-        }
-        if (active) {
-            if (pc < kk) y[yrow + pc] = a0;
-            if (pc + 1 < kk) y[yrow + pc + 1] = a1;
-        }
-    });
+        for (int s = 0; s < kRowSets; ++s) v[s] = rows[std::uint64_t(s) * stride + p];
+        y[p] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    }
 }
 
 // Pass-1 waves per SIMD the register allocation must allow, by right-hand
@@ -1161,17 +1120,20 @@ constexpr int pass1_occupancy(int kb) { return kb <= 1 ? 4 : 2; }
 constexpr int batch1_occupancy(int kb) { return kb <= 1 ? 3 : 2; }
 
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kWaves / kPass1Groups, pass1_occupancy(KB)) void apply_pass1(
+__global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_pass1(
     MemberView M) {
-    __shared__ __align__(16) std::uint8_t lds[kWaves / kPass1Groups * wave_lds_bytes<KB>()];
-    pass1_group<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, lds);
+    __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
+    group_rows<KB, FP32, 1>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
-template <int KB, bool FP32, int W>
-__global__ __launch_bounds__(64 * W) void apply_pass2(MemberView M) {
-    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
-    pass2_tile<KB, FP32, W>(M, blockIdx.x, blockIdx.y * KB, lds);
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_pass2(
+    MemberView M) {
+    __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
+    group_rows<KB, FP32, 2>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
+
+__global__ __launch_bounds__(64) void apply_combine(MemberView M) { combine_tile(M, blockIdx.x); }
 
 // The member owning global work index `index`: first[m] <= index < first[m+1].
 __device__ __forceinline__ std::uint32_t locate_member(const std::uint32_t* __restrict__ first,
@@ -1197,30 +1159,26 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
     return M;
 }
 
-template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kWaves / kPass1Groups, batch1_occupancy(KB)) void batch_pass1(
+template <int KB, bool FP32, int PASS>
+__global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_pass(
     const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
     std::uint32_t count, const double* x, double* y, double* zp) {
-    __shared__ __align__(16) std::uint8_t lds[kWaves / kPass1Groups * wave_lds_bytes<KB>()];
+    __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
     const std::uint32_t g = blockIdx.x / kPass1Groups;
     const std::uint32_t m = locate_member(first, count, g);
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
-    pass1_group<KB, FP32>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
+    group_rows<KB, FP32, PASS>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
 }
 
-template <int KB, bool FP32, int W>
-__global__ __launch_bounds__(64 * W) void batch_pass2(const MemberView* __restrict__ members,
-                                                      const std::uint32_t* __restrict__ first,
-                                                      std::uint32_t count, const double* x,
-                                                      double* y, double* zp) {
-    __shared__ __align__(16) std::uint8_t lds[W * wave_lds_bytes<KB>()];
+__global__ __launch_bounds__(64) void batch_combine(const MemberView* __restrict__ members,
+                                                    const std::uint32_t* __restrict__ first,
+                                                    std::uint32_t count, const double* x,
+                                                    double* y, double* zp) {
     const std::uint32_t m = locate_member(first, count, blockIdx.x);
     const MemberView M = bind_member(members, m, x, y, zp);
-    const int p0 = blockIdx.y * KB;
-    if (p0 >= M.k) return;
-    pass2_tile<KB, FP32, W>(M, blockIdx.x - first[m], p0, lds);
+    combine_tile(M, blockIdx.x - first[m]);
 }
 
 // Test hook: codes of one factor side of one block, from its chunks, written
@@ -1372,39 +1330,39 @@ int rhs_block(const DeviceStore& s, std::int64_t k) {
     return it == s.rhs_block.end() ? default_rhs_block(k) : it->second;
 }
 
-constexpr int waves_for(int kb) { return kb >= 2 ? 8 : 16; }
-
 template <int KB, bool FP32>
 void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
                   std::int64_t ldy, double* zp, hipStream_t stream) {
-    constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((k + KB - 1) / KB);
     const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
     hipLaunchKernelGGL((apply_pass1<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
-                       dim3(64 * kWaves / kPass1Groups), 0, stream, M);
+                       dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
-    hipLaunchKernelGGL((apply_pass2<KB, FP32, W>), dim3(s.n_tiles, passes), dim3(64 * W), 0,
-                       stream, M);
+    hipLaunchKernelGGL((apply_pass2<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
+                       dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 2 launch");
+    hipLaunchKernelGGL(apply_combine, dim3(s.n_tiles), dim3(64), 0, stream, M);
+    hip_check(hipGetLastError(), "apply combine launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
 }
 
 template <int KB, bool FP32>
 void launch_batch_class(const BatchClass& c, const double* x, double* y, double* zp,
                         hipStream_t stream) {
-    constexpr int W = waves_for(KB);
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
-    hipLaunchKernelGGL((batch_pass1<KB, FP32>), dim3(c.total_tiles * kPass1Groups, passes),
-                       dim3(64 * kWaves / kPass1Groups), 0, stream, c.members, c.first_tile, count,
-                       x, y, zp);
+    hipLaunchKernelGGL((batch_pass<KB, FP32, 1>), dim3(c.total_tiles * kPass1Groups, passes),
+                       dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 1 launch");
-    hipLaunchKernelGGL((batch_pass2<KB, FP32, W>), dim3(c.total_tiles, passes), dim3(64 * W), 0,
-                       stream, c.members, c.first_tile, count, x, y, zp);
+    hipLaunchKernelGGL((batch_pass<KB, FP32, 2>), dim3(c.total_tiles * kPass1Groups, passes),
+                       dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 2 launch");
+    hipLaunchKernelGGL(batch_combine, dim3(c.total_tiles), dim3(64), 0, stream, c.members,
+                       c.first_tile, count, x, y, zp);
+    hip_check(hipGetLastError(), "batched combine launch");
 }
 
 template <bool FP32>
@@ -2072,7 +2030,7 @@ void destroy_device_store(DeviceStore* s) {
 }
 
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k) {
-    return (s.z_rows + kPass1Groups * s.order) * std::uint64_t(k) * sizeof(double);
+    return (s.z_rows + kRowSets * s.order) * std::uint64_t(k) * sizeof(double);
 }
 
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,

@@ -69,11 +69,11 @@ struct ItemDev {
 static_assert(sizeof(ItemDev) == 32, "ItemDev is 32 bytes");
 
 // Row tile: rows [row0, row0+rows).  Run w (of kWaves) of the tile's pass-1
-// workgroup streams the kRight items [rsplit[w], rsplit[w+1]) (a share of
+// workgroups streams the kRight items [rsplit[w], rsplit[w+1]) (a share of
 // all right factors, independent of the tile) and the tile's items
 // [split1[w], split1[w+1]); run w of pass 2 the items [split2[w],
-// split2[w+1]).  A launch with W < kWaves waves gives each wave kWaves/W
-// consecutive runs.
+// split2[w+1]).  Each pass runs a tile's 16 runs as four workgroups of four
+// waves.
 struct TileDev {
     std::uint32_t row0, rows;
     std::uint32_t rsplit[kWaves + 1];
@@ -94,8 +94,8 @@ struct MemberView {
     const TileDev* tiles;
     const double* x;
     double* y;
-    double* zp;     // workspace: z_rows x k partial sums, then kPass1Groups sets of
-                    // order x k row sums
+    double* zp;     // workspace: z_rows x k partial sums, then 2 kPass1Groups sets of
+                    // order x k row sums (four per pass)
     std::int64_t ldx, ldy;
     std::int32_t k;
     std::uint32_t tiles_count;  // row tiles
@@ -143,7 +143,7 @@ struct DeviceStore {
     // sizes
     std::uint64_t n_words = 0, n_comp = 0;
     std::uint32_t n_tiles = 0, n_items = 0;
-    std::uint64_t z_rows = 0;  // workspace = (z_rows + kPass1Groups * order) * k doubles
+    std::uint64_t z_rows = 0;  // workspace = (z_rows + 2 kPass1Groups order) k doubles
     // host mirrors needed by entry points
     std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;  // per low-rank block
