@@ -162,10 +162,11 @@ qh_status qh_graph_launch(const qh_graph *graph, void *stream);
 qh_status qh_graph_destroy(qh_graph *graph);
 
 /* Per-pass timing of qh_apply on the stream the passes are launched on.
- * qh_profile_begin arms hipEvents around each of the two passes of the next
+ * qh_profile_begin arms hipEvents around the launches of the next
  * `max_applies` calls on this store (the events are created here, not in the
  * timed calls); qh_profile_end waits for the recorded events, returns the
- * totals and disarms.  Pass 0 = right-factor pass, pass 1 = row-tile pass. */
+ * totals and disarms.  pass_ms[0] = pass 1 (right factors, diagonal and fused
+ * blocks), pass_ms[1] = pass 2 (left factors) and the combine launch. */
 typedef struct {
     uint64_t applies;
     double pass_ms[2];

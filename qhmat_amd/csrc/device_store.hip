@@ -1096,20 +1096,21 @@ __device__ __forceinline__ void group_rows(const MemberView& M, std::uint32_t gr
     });
 }
 
-// y rows of tile t: the tree over the eight row sets (lanes = rows).
+// y rows of tile t: the tree over the eight row sets, one thread per
+// (row, right-hand side) so that reads and writes run along the columns.
 __device__ __forceinline__ void combine_tile(const MemberView& M, std::uint32_t t) {
-    const int lane = threadIdx.x & 63;
     const TileDev* __restrict__ tile = M.tiles + t;
     const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
-    if (std::uint32_t(lane) >= R) return;
-    const std::uint64_t k = std::uint64_t(M.k), stride = std::uint64_t(M.order) * k;
-    const double* rows = M.zp + (M.z_rows + tile->row0 + lane) * k;
-    double* y = M.y + std::int64_t(M.perm[tile->row0 + lane]) * M.ldy;
-    for (std::uint64_t p = 0; p < k; ++p) {
+    const std::uint32_t k = std::uint32_t(M.k);
+    const std::uint64_t stride = std::uint64_t(M.order) * k;
+    const double* rows = M.zp + (M.z_rows + tile->row0) * std::uint64_t(k);
+    for (std::uint32_t e = threadIdx.x; e < R * k; e += blockDim.x) {
+        const std::uint32_t i = e / k, p = e - i * k;
         double v[kRowSets];
 #pragma unroll
-        for (int s = 0; s < kRowSets; ++s) v[s] = rows[std::uint64_t(s) * stride + p];
-        y[p] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        for (int s = 0; s < kRowSets; ++s) v[s] = rows[std::uint64_t(s) * stride + e];
+        M.y[std::int64_t(M.perm[tile->row0 + i]) * M.ldy + p] =
+            ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
 }
 
@@ -1133,7 +1134,7 @@ __global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_p
     group_rows<KB, FP32, 2>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
-__global__ __launch_bounds__(64) void apply_combine(MemberView M) { combine_tile(M, blockIdx.x); }
+__global__ __launch_bounds__(256) void apply_combine(MemberView M) { combine_tile(M, blockIdx.x); }
 
 // The member owning global work index `index`: first[m] <= index < first[m+1].
 __device__ __forceinline__ std::uint32_t locate_member(const std::uint32_t* __restrict__ first,
@@ -1172,7 +1173,7 @@ __global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_
     group_rows<KB, FP32, PASS>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
 }
 
-__global__ __launch_bounds__(64) void batch_combine(const MemberView* __restrict__ members,
+__global__ __launch_bounds__(256) void batch_combine(const MemberView* __restrict__ members,
                                                     const std::uint32_t* __restrict__ first,
                                                     std::uint32_t count, const double* x,
                                                     double* y, double* zp) {
@@ -1344,7 +1345,7 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     hipLaunchKernelGGL((apply_pass2<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 2 launch");
-    hipLaunchKernelGGL(apply_combine, dim3(s.n_tiles), dim3(64), 0, stream, M);
+    hipLaunchKernelGGL(apply_combine, dim3(s.n_tiles), dim3(256), 0, stream, M);
     hip_check(hipGetLastError(), "apply combine launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
 }
@@ -1360,7 +1361,7 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
     hipLaunchKernelGGL((batch_pass<KB, FP32, 2>), dim3(c.total_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 2 launch");
-    hipLaunchKernelGGL(batch_combine, dim3(c.total_tiles), dim3(64), 0, stream, c.members,
+    hipLaunchKernelGGL(batch_combine, dim3(c.total_tiles), dim3(256), 0, stream, c.members,
                        c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched combine launch");
 }
