@@ -61,15 +61,19 @@ def algorithmic_bytes(stats: dict, k: int) -> dict:
     stores them (per block and component, not per tile), two FP64 scales and
     a width per component, the diagonal blocks, x and y once and the
     permutation (8 bytes per index, as the reference holds it).  Split by
-    pass: pass 1 reads the right factors, the fused blocks' left factors, the
-    diagonal blocks and x; pass 2 the remaining left factors and writes y."""
+    pass: pass 1 reads the right factors (and their widths), its share of the
+    fused blocks and diagonal blocks, and x; pass 2 the remaining left factors
+    (with the scales) and its share of the fused and diagonal blocks, and
+    writes y."""
     n = stats["order"]
     comps = stats["ncomponents"]
     fused = stats["fused_components"]
     w1, w2 = stats["pass_code_words"]
+    f1, f2 = stats["pass_fused_components"]
+    d1, d2 = stats["pass_dense_values"]
     total = 4 * stats["code_words"] + 20 * comps + 8 * stats["dense_values"] + 16 * n * k + 8 * n
-    pass1 = 4 * w1 + 20 * fused + 4 * (comps - fused) + 8 * stats["dense_values"] + 8 * n * k
-    pass2 = 4 * w2 + 16 * (comps - fused) + 8 * n * k + 8 * n
+    pass1 = 4 * w1 + 20 * f1 + 4 * (comps - fused) + 8 * d1 + 8 * n * k
+    pass2 = 4 * w2 + 20 * f2 + 16 * (comps - fused) + 8 * d2 + 8 * n * k + 8 * n
     assert pass1 + pass2 == total
     return {"total": total, "code_stream": 4 * stats["code_words"], "pass0": pass1,
             "pass1": pass2}

@@ -93,7 +93,10 @@ typedef struct {
     uint64_t items;               /* work items of both passes                */
     uint64_t right_waves;         /* pass-1 waves streaming right factors     */
     uint64_t pass_code_words[2];  /* of code_words: read by pass 1 / pass 2    */
-    uint64_t fused_components;    /* components of blocks applied in pass 1   */
+    uint64_t fused_components;    /* components of blocks in one row and one
+                                     column tile (both factors in one item)   */
+    uint64_t pass_fused_components[2]; /* of fused_components: pass 1 / 2     */
+    uint64_t pass_dense_values[2];     /* of dense_values: pass 1 / pass 2    */
     int32_t device;
 } qh_store_stats;
 

@@ -60,12 +60,14 @@ class StoreStats(C.Structure):
         ("dense_values", _u64), ("device_bytes", _u64),
         ("lowrank_flops_per_rhs", _u64), ("dense_flops_per_rhs", _u64),
         ("tiles", _u64), ("items", _u64), ("right_waves", _u64),
-        ("pass_code_words", _u64 * 2), ("fused_components", _u64), ("device", _i32),
+        ("pass_code_words", _u64 * 2), ("fused_components", _u64),
+        ("pass_fused_components", _u64 * 2), ("pass_dense_values", _u64 * 2), ("device", _i32),
     ]
 
     def as_dict(self) -> dict:
         out = {name: getattr(self, name) for name, _ in self._fields_}
-        out["pass_code_words"] = list(self.pass_code_words)
+        for name in ("pass_code_words", "pass_fused_components", "pass_dense_values"):
+            out[name] = list(getattr(self, name))
         return out
 
 

@@ -124,12 +124,14 @@ template <bool FP32> using real_t = typename std::conditional<FP32, float, doubl
 // of a stream may read (64 codes of 32 bits and the word after), then the
 // item's inputs (x rows of its columns, or the workspace partial sums of
 // kLeft), KB values per lane, which also take the item's rows at its end,
-// then the scaled inputs of 16 components (zt).
+// then the scaled inputs of 16 components (zt), then the item's scale
+// products and widths (lanes = components).
 constexpr int kStageBytes = kItemQuads * 16 + 272;
 constexpr int kLdsHead = 16;  // readable words before the stage (decode_end)
-template <int KB> constexpr int wave_lds_bytes() {
+template <int KB> constexpr int wave_lds_meta() {
     return kLdsHead + kStageBytes + kTile * KB * 8 + 16 * KB * 8;
 }
+template <int KB> constexpr int wave_lds_bytes() { return wave_lds_meta<KB>() + 64 * 12; }
 
 // Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
 // items of the group).
@@ -766,19 +768,21 @@ __device__ __forceinline__ void right_rows(const MemberView& M, const Item& f,
     }
 }
 
-// The registers of an item that outlive its staging: component widths and
-// scale products (lanes = components).
+// Component widths and scale products of the staged item (lanes =
+// components), read back from LDS after the staging.
 struct Staged {
     std::uint32_t bits_l;
     double scale_l;
 };
 
-// Moves a fetched item into the wave's LDS: its data, and its x rows (lanes =
+// Moves a fetched item into the wave's LDS: its data, its x rows (lanes =
 // columns; zero past the item's columns) or kLeft partial sums (lanes =
-// pairs; zero past them), and keeps the rest.
+// pairs; zero past them), and its widths and scale products.  Every
+// register of the fetch buffer is dead afterwards, so the next fetch into it
+// needs no copy of a load still in flight.
 template <int KB>
-__device__ __forceinline__ Staged stage_item(const Item& f, const Fetch<KB>& F, int lane,
-                                             std::uint8_t* wave_lds) {
+__device__ __forceinline__ void stage_item(const Item& f, const Fetch<KB>& F, int lane,
+                                           std::uint8_t* wave_lds) {
     quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds + kLdsHead);
     double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
 #pragma unroll
@@ -786,7 +790,13 @@ __device__ __forceinline__ Staged stage_item(const Item& f, const Fetch<KB>& F, 
         if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
 #pragma unroll
     for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
-    return Staged{F.bits, F.scale};
+    reinterpret_cast<double*>(wave_lds + wave_lds_meta<KB>())[lane] = F.scale;
+    reinterpret_cast<std::uint32_t*>(wave_lds + wave_lds_meta<KB>() + 512)[lane] = F.bits;
+}
+template <int KB>
+__device__ __forceinline__ Staged staged_meta(const std::uint8_t* wave_lds, int lane) {
+    return Staged{reinterpret_cast<const std::uint32_t*>(wave_lds + wave_lds_meta<KB>() + 512)[lane],
+                  reinterpret_cast<const double*>(wave_lds + wave_lds_meta<KB>())[lane]};
 }
 
 // A kFused, kRight (pass 1) or kLeft (pass 2) item, 16 components at a
@@ -956,7 +966,7 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
 // (kDense, kFused, kLeft) are added to acc (lanes = the R rows of the tile);
 // kRight items write their partial sums to the workspace.  Two items are in
 // flight while one is decoded: the fetch buffers alternate, and the
-// permutation rows of the next item to fetch are requested one step ahead.
+// permutation rows of an item are requested two steps before its fetch.
 template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1, std::uint32_t e1,
                                           std::uint32_t b2, std::uint32_t e2, int p0, int kk,
@@ -979,38 +989,52 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
         // the loads in flight are counted exactly and never waited for early.
         const Item none = {};
         auto slot = [&](std::uint32_t i) { return i < n ? item_at(r0, r1, i) : none; };
-        Item fa = slot(0), fb = slot(1), fc = slot(2);
+        Item fa = slot(0), fb = slot(1), fc = slot(2), fd = slot(3);
         Fetch<KB> FA, FB;
-        fetch<KB>(M, fa, perm_row(M, fa, lane), lane, p0, kk, FA);
-        fetch<KB>(M, fb, perm_row(M, fb, lane), lane, p0, kk, FB);
-        std::int32_t pc = perm_row(M, fc, lane);  // rows of the item fetched next
+        // Permutation rows of the next fetch into buffer A (PA) and into B
+        // (PB), requested two steps before that fetch.
+        std::int32_t PA = perm_row(M, fa, lane), PB = perm_row(M, fb, lane);
+        fetch<KB>(M, fa, PA, lane, p0, kk, FA);
+        PA = perm_row(M, fc, lane);
+        fetch<KB>(M, fb, PB, lane, p0, kk, FB);
+        PB = perm_row(M, fd, lane);
 
-        // steady state: buffer A holds item s, buffer B item s+1
+        // Steady state: buffer A holds item s, buffer B item s+1.  Step s
+        // waits for item s (fetched two steps earlier), fetches item s+2 with
+        // rows requested two steps earlier and requests the rows of item s+4;
+        // neither wait covers item s+1, so two items' data stay in flight
+        // during a decode.
         for (std::uint32_t s = 0; s < n; s += 2) {
             {
-                const Staged st = stage_item<KB>(fa, FA, lane, wave_lds);
-                fetch<KB>(M, fc, pc, lane, p0, kk, FA);
-                const Item fd = slot(s + 3);
-                const std::int32_t pd = perm_row(M, fd, lane);
+                stage_item<KB>(fa, FA, lane, wave_lds);
+                fetch<KB>(M, fc, PA, lane, p0, kk, FA);
+                const Item fe = slot(s + 4);
+                PA = perm_row(M, fe, lane);
                 wave_lds_sync();
-                compute_item<KB, FP32, PASS>(M, fa, st, lane, p0, kk, R, wave_lds, acc);
+                compute_item<KB, FP32, PASS>(M, fa, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
+                                             R, wave_lds, acc);
                 wave_lds_sync();  // the next item overwrites the stage
                 fa = fc;
-                fc = fd;
-                pc = pd;
+                fc = fe;
             }
-            if (s + 1 >= n) break;
+            // Narrow passes take no exit here: an odd count runs one empty
+            // item, whose loads all read zero, so that both steps always run
+            // and the waits for loads in flight stay exact across the loop.
+            // Wide passes exit (one path less keeps their registers in bounds).
+            if constexpr (KB >= kMatrixMinKB) {
+                if (s + 1 >= n) break;
+            }
             {
-                const Staged st = stage_item<KB>(fb, FB, lane, wave_lds);
-                fetch<KB>(M, fc, pc, lane, p0, kk, FB);
-                const Item fd = slot(s + 4);
-                const std::int32_t pd = perm_row(M, fd, lane);
+                stage_item<KB>(fb, FB, lane, wave_lds);
+                fetch<KB>(M, fd, PB, lane, p0, kk, FB);
+                const Item fe = slot(s + 5);
+                PB = perm_row(M, fe, lane);
                 wave_lds_sync();
-                compute_item<KB, FP32, PASS>(M, fb, st, lane, p0, kk, R, wave_lds, acc);
+                compute_item<KB, FP32, PASS>(M, fb, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
+                                             R, wave_lds, acc);
                 wave_lds_sync();
-                fb = fc;
-                fc = fd;
-                pc = pd;
+                fb = fd;
+                fd = fe;
             }
         }
     }
@@ -1400,10 +1424,23 @@ template <class T> T* upload(const std::vector<T>& host, std::uint64_t& bytes) {
     return p;
 }
 
-// Relative cost of an item for balancing waves: its bytes plus a fixed
-// share for the records, scans and the pipeline turn.
-constexpr std::uint64_t kItemCost = 512;
-std::uint64_t item_cost(const ItemDev& it) { return kItemCost + 16 * std::uint64_t(it.quads); }
+// Relative time of an item for balancing runs, from per-wave clock
+// readings of the narrow passes (one right-hand side) fitted over a 2-D
+// kernel matrix of order 16384: a fixed part per item (records, staging,
+// scans, the pipeline turn) plus a part per component, more for components
+// wider than 24 bits (split in FP32 mode).  A diagonal item costs little
+// beside its bytes.  Units: hundredths of the per-component time.
+std::uint64_t item_cost(const ItemDev& it, const std::vector<std::int32_t>& widths) {
+    if (it.kind == kDense) return 7 * std::uint64_t(it.quads);
+    std::uint64_t wide = 0;
+    for (std::uint32_t j = 0; j < it.count; ++j) wide += widths[it.comp0 + j] > 24 ? 1 : 0;
+    const std::uint64_t n = it.count;
+    switch (it.kind) {
+    case kFused: return 1720 + 164 * n + 94 * wide;
+    case kRight: return 1060 + 147 * n + 94 * wide;
+    default: return 1340 + 123 * n + 62 * wide;  // kLeft
+    }
+}
 
 // Tiles of at most kTile indices between consecutive cuts.
 std::vector<std::pair<std::uint64_t, std::uint64_t>> make_tiles(std::vector<std::uint64_t> cuts) {
@@ -1738,19 +1775,28 @@ Plan build_plan(const qh_store_desc& d) {
             }
     }
 
+    // Pass assignment: kDense, kFused and kRight items run in pass 1 (kRight
+    // in any run, the others in their tile's runs), kLeft items in pass 2.
+    auto cost_of = [&](const ItemDev& it) { return item_cost(it, plan.widths); };
+    const std::size_t ntiles = row_tiles.size();
+    std::uint64_t right_load = 0;
+    for (const auto& it : right_items) right_load += cost_of(it);
+    const std::uint64_t pass_fused[2] = {fused_components, 0};
+    const std::uint64_t pass_dense[2] = {dense_values, 0};
+
     // Item table: right-factor items, then per tile its pass-1 and pass-2 items.
     std::vector<ItemDev>& items = plan.items;
     items = right_items;
-    plan.tiles.resize(row_tiles.size());
+    plan.tiles.resize(ntiles);
     std::vector<std::uint64_t> run_cost;  // pass-1 cost of every (tile, run)
-    for (std::size_t t = 0; t < row_tiles.size(); ++t) {
+    for (std::size_t t = 0; t < ntiles; ++t) {
         TileDev& T = plan.tiles[t];
         T.row0 = std::uint32_t(row_tiles[t].first);
         T.rows = std::uint32_t(row_tiles[t].second - row_tiles[t].first);
         for (int pass = 0; pass < 2; ++pass) {
             const auto& list = pass == 0 ? pass1[t] : pass2[t];
             std::vector<std::uint64_t> cost;
-            for (const auto& it : list) cost.push_back(item_cost(it));
+            for (const auto& it : list) cost.push_back(cost_of(it));
             std::uint32_t* split = pass == 0 ? T.split1 : T.split2;
             split_runs(cost, kWaves, std::uint32_t(items.size()), split);
             if (pass == 0)
@@ -1765,21 +1811,24 @@ Plan build_plan(const qh_store_desc& d) {
     }
     require(items.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
     // The right-factor items fill the pass-1 runs up to an even total: run r
-    // takes items (in order) while its cost stays under the mean.
+    // takes items (in order) while the cost of runs 0..r stays under r + 1
+    // means, so that rounding does not pile up in the last run.
     std::uint64_t all = 0;
     for (auto c : run_cost) all += c;
-    for (const auto& it : right_items) all += item_cost(it);
-    const std::uint64_t mean = all / std::max<std::size_t>(1, run_cost.size());
+    all += right_load;
+    const std::uint64_t nruns = std::max<std::uint64_t>(1, run_cost.size());
     std::uint32_t next = 0;
+    std::uint64_t done = 0;  // cost of the runs filled so far
     for (std::size_t r = 0; r < run_cost.size(); ++r) {
         TileDev& T = plan.tiles[r / kWaves];
         const int w = int(r % kWaves);
         T.rsplit[w] = next;
-        std::uint64_t c = run_cost[r];
+        done += run_cost[r];
+        const std::uint64_t goal = all * (r + 1) / nruns;
         const bool last = r + 1 == run_cost.size();
         while (next < right_items.size() &&
-               (last || c + item_cost(right_items[next]) / 2 <= mean)) {
-            c += item_cost(right_items[next]);
+               (last || done + cost_of(right_items[next]) / 2 <= goal)) {
+            done += cost_of(right_items[next]);
             ++next;
         }
         T.rsplit[w + 1] = next;
@@ -1810,6 +1859,10 @@ Plan build_plan(const qh_store_desc& d) {
     st.pass_code_words[0] = pass_words[0];
     st.pass_code_words[1] = pass_words[1];
     st.fused_components = fused_components;
+    for (int p = 0; p < 2; ++p) {
+        st.pass_fused_components[p] = pass_fused[p];
+        st.pass_dense_values[p] = pass_dense[p];
+    }
     plan.perm = std::move(perm);
     return plan;
 }
@@ -1971,7 +2024,7 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
                 for (std::uint32_t e = split[w]; e < split[w + 1]; ++e) {
                     const ItemDev& it = at(pl.items, e);
                     check_item(it);
-                    if (pass == 0 && it.kind == kDense) {
+                    if (it.kind == kDense) {
                         check(it.count <= kDenseColumns, "dense item columns");
                         for (std::uint32_t c = 0; c < it.count; ++c)
                             for (std::uint32_t i = 0; i < R; ++i) {
@@ -1984,7 +2037,7 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
                                 for (std::uint64_t p = 0; p < K; ++p)
                                     acc[i * K + p] += v * xval(it.col + c, p);
                             }
-                    } else if (pass == 0 && it.kind == kFused) {
+                    } else if (it.kind == kFused) {
                         const std::uint64_t roff = right_sums(it, z);
                         left_rows(it, roff, R, z, acc);
                     } else if (pass == 1 && it.kind == kLeft) {
@@ -1999,7 +2052,7 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
                                   it.count <= pl.z_rows, "left item reads past its partial sums");
                         left_rows(it, 0, R, z, acc);
                     } else {
-                        throw std::out_of_range("item kind in the wrong pass");
+                        throw std::out_of_range("kRight item in a tile's run");
                     }
                 }
                 for (std::size_t i = 0; i < total.size(); ++i) total[i] += acc[i];
