@@ -74,21 +74,18 @@ void require(bool ok, const char* message) {
 // ---------------------------------------------------------------------------
 // Device helpers
 
-// Code number `idx` of the stream at `seg` (LDS or global).  The word pair is
-// always read: a code that does not straddle takes all its bits from the low
-// word, and one word past every staged stream is kept readable.
-__device__ __forceinline__ int decode_code(const std::uint32_t* seg, std::uint32_t idx,
-                                           std::uint32_t bits) {
-    const std::uint32_t pos = idx * bits;
-    const std::uint32_t w = pos >> 5;
-    const std::uint32_t raw = __builtin_amdgcn_alignbit(seg[w + 1], seg[w], pos & 31u);
+// Code of lane `lane` at bit offset `o` of a side block at `side` (word w of
+// a lane at 64 w + lane; global memory).  The next row of the block is read
+// only when the code straddles into it.
+__device__ __forceinline__ int decode_side_code(const std::uint32_t* side, std::uint32_t lane,
+                                                std::uint32_t o, std::uint32_t bits) {
+    const std::uint32_t k = o >> 5, sh = o & 31u;
+    const std::uint32_t lo = side[64u * k + lane];
+    const std::uint32_t hi = sh + bits > 32u ? side[64u * (k + 1u) + lane] : 0u;
+    const std::uint32_t raw = __builtin_amdgcn_alignbit(hi, lo, sh);
     return bits == 32 ? static_cast<int>(raw) : __builtin_amdgcn_sbfe(raw, 0u, bits);
 }
 
-__host__ __device__ __forceinline__ std::uint32_t stream_words(std::uint32_t count,
-                                                               std::uint32_t bits) {
-    return (count * bits + 31u) >> 5;
-}
 
 // LDS written by some lanes of a wave and read by others: keep the compiler
 // from moving accesses across this point and wait for the LDS queue.
@@ -127,7 +124,7 @@ template <bool FP32> using real_t = typename std::conditional<FP32, float, doubl
 // then the scaled inputs of 16 components (zt), then the item's scale
 // products and widths (lanes = components).
 constexpr int kStageBytes = kItemQuads * 16 + 272;
-constexpr int kLdsHead = 16;  // readable words before the stage (decode_end)
+constexpr int kLdsHead = 256;  // readable bytes before the stage (decode_lm)
 template <int KB> constexpr int wave_lds_meta() {
     return kLdsHead + kStageBytes + kTile * KB * 8 + 16 * KB * 8;
 }
@@ -232,12 +229,22 @@ __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::i
         F.in[p] = load_f64(r, (mine && p < kk) ? (row + std::uint32_t(p0 + p)) * 8u : kOutOfRange);
 }
 
-// Per-component uniform fields packed in one lane value: stream offset in
-// words (< 2^16) and width.  Lanes past the item's components hold a dummy
-// (offset = end of the streams, 1 bit) whose input is zero, so loops over
-// whole groups of components need no tests.
-__device__ __forceinline__ std::uint32_t pack_meta(bool comp, std::uint32_t off, std::uint32_t bits) {
-    return off | ((comp ? bits : 1u) << 16);
+// Per-component fields packed in one lane value (lanes = the item's
+// components): the end bit E of the component's code in every lane's stream
+// of a side block (the inclusive prefix sum of the widths) and its width b,
+// as 64 E + b.  Lanes past the item's components hold a dummy (E = 32,
+// b = 1: the top bit of a lane's first word) whose input is zero, so loops
+// over whole groups of components need no tests.  `total` receives the bits
+// per lane of the side block.
+__device__ __forceinline__ std::uint32_t side_meta(bool comp, std::uint32_t bits, int lane,
+                                                   std::uint32_t& total) {
+    const std::uint32_t E = wave_scan(comp ? bits : 0u, lane);
+    total = __builtin_amdgcn_readlane(E, 63);
+    return comp ? E * 64u + bits : 32u * 64u + 1u;
+}
+// Words of a side block of `total` bits per lane.
+__device__ __forceinline__ std::uint32_t side_words(std::uint32_t total) {
+    return 64u * ((total + 31u) >> 5);
 }
 
 // Matrix cores.  v_mfma_f32_16x16x4f32 and v_mfma_f64_16x16x4f64 form
@@ -274,15 +281,18 @@ template <bool FP32> __host__ __device__ __forceinline__ int a_row(int r) {
     return FP32 ? r : 4 * (r & 3) + (r >> 2);
 }
 
-// The b-bit code that ends just below bit E of the staged words (one readable
-// word is kept before the stage): the 32 bits [E - 32, E) from the word pair
-// (v_alignbit_b32), shifted arithmetically right by sh = 32 - b, which
-// sign-extends it (b = 32 included).  Bits below the code's stream are
-// shifted out, so no branch and no mask.
-__device__ __forceinline__ int decode_end(const std::uint32_t* stage, std::uint32_t E,
-                                          std::uint32_t sh) {
-    const std::uint32_t* w = stage + (E >> 5);
-    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-1], E)) >> sh;
+// The code of lane `lane` for the component with packed fields m in a staged
+// side block (word w of lane L at side[64 w + L]; one row of 64 readable
+// words is kept before the stage and after the item): the 32 bits [E - 32,
+// E) of the lane's stream from words (E >> 5) - 1 and E >> 5
+// (v_alignbit_b32), shifted arithmetically right by 32 - b, which
+// sign-extends the field (b = 32 included) and drops the bits below it.  No
+// mask and no branch; every lane reads its own column of words, so the 64
+// lanes of a read hit 64 consecutive words.
+__device__ __forceinline__ int decode_lm(const std::uint32_t* side, std::uint32_t m, int lane) {
+    const std::uint32_t E = m >> 6;
+    const std::uint32_t* w = side + 64u * (E >> 5) + std::uint32_t(lane);
+    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-64], E)) >> (32u - (m & 63u));
 }
 
 // A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
@@ -309,18 +319,6 @@ __device__ __forceinline__ double lane_value(double v, std::uint32_t j) {
     const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), j));
     const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), j));
     return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
-// Code idx1 - 1 of a stream whose first word is at `base` (LDS, with the
-// word before it readable): the 32 bits that end with the code's last bit
-// (v_alignbit_b32 of the word pair), shifted arithmetically right by 32 - b,
-// which sign-extends the field (b = 32 included) and drops the bits below
-// it.  No branches: lanes past the end of a stream read the stage's slack.
-__device__ __forceinline__ int decode_lane(const std::uint32_t* base, std::uint32_t bits,
-                                           std::uint32_t idx1) {
-    const std::uint32_t end = __umul24(idx1, bits);
-    const std::uint32_t* w = base + __builtin_amdgcn_ubfe(end, 5u, 27u);  // end / 32
-    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-1], end)) >> (32u - bits);
 }
 
 // Cross-lane moves without LDS: DPP within rows of 16 lanes, and the gfx950
@@ -444,7 +442,7 @@ __device__ __forceinline__ void valu_left_four(const std::uint32_t* stage, const
                                           double (&acc)[KB], float (&blk)[KB]) {
     int code[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) code[u] = decode_lane(stage + (meta[u] & 0xFFFFu), meta[u] >> 16, lane + 1);
+    for (int u = 0; u < 4; ++u) code[u] = decode_lm(stage, meta[u], lane);
     if (!FP32) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -518,7 +516,7 @@ __device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std
 #pragma unroll
     for (int g = 0; g < N; ++g) {
         const std::uint32_t meta = __builtin_amdgcn_readlane(rmeta_l, min(j0 + g, 63u));
-        code[g] = decode_lane(stage + (meta & 0xFFFFu), meta >> 16, lane + 1);
+        code[g] = decode_lm(stage, meta, lane);
     }
     if (!FP32 || !wide) {
 #pragma unroll
@@ -545,13 +543,14 @@ __device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std
     }
 }
 
-// One block of N components of a kRight or kFused item: right-factor sums,
-// then (kRight) their workspace rows or (kFused) the scaled inputs and the
-// left streams of the same components.
+// One block of N components of a kRight or kFused item: right-factor sums
+// from the right side block at `stage`, then (kRight) their workspace rows
+// or (kFused) the scaled inputs and the left side block at `lstage` (the
+// same components, so the same packed fields).
 template <int N, int KB, bool FP32>
 __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const Item& f,
                                                 const std::uint32_t* stage, std::uint32_t rmeta_l,
-                                                std::uint32_t lmeta_l, std::uint64_t wide_mask,
+                                                const std::uint32_t* lstage, std::uint64_t wide_mask,
                                                 double scale_l, std::uint32_t j0, int lane, int p0,
                                                 int kk, const real_t<FP32> (&xl)[KB],
                                                 double (&acc)[KB], float (&blk)[KB]) {
@@ -572,7 +571,7 @@ __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const 
     Real zs_lane[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zs_lane[p] = jl < f.count ? Real(sc * tot[p]) : Real(0);
-    valu_left_rows<KB, FP32>(stage, lmeta_l, wide_mask, j0, min(j0 + N, f.count), lane,
+    valu_left_rows<KB, FP32>(lstage, rmeta_l, wide_mask, j0, min(j0 + N, f.count), lane,
                         [&](std::uint32_t j, int u, int p) {
                             return lane_value(zs_lane[p], block_lane<N>(int(j + u - j0)));
                         },
@@ -609,24 +608,22 @@ __device__ __forceinline__ void left_group(const std::uint32_t* stage, const rea
     using Mm = Mma<FP32>;
     using Real = real_t<FP32>;
     const int q = lane >> 4, r = lane & 15;
-    const std::uint32_t row1 = std::uint32_t(a_row<FP32>(r)) + 1u;
+    const int row = a_row<FP32>(r);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const std::uint32_t c = g0 + 4u * std::uint32_t(s) + std::uint32_t(q);  // <= 63
         const std::uint32_t meta = std::uint32_t(__shfl(int(lmeta_l), int(c), 64));
-        const std::uint32_t bits = meta >> 16, sh = 32u - bits;
-        const std::uint32_t E0 = (meta & 0xFFFFu) * 32u + row1 * bits, step = 16u * bits;
         const Real zs = zt[(4 * s + q) * KB + (r & (KB - 1))];
         if constexpr (!WIDE) {
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
-                rows.d[rb] = Mm::mad(Real(decode_end(stage, E0 + rb * step, sh)), zs, rows.d[rb]);
+                rows.d[rb] = Mm::mad(Real(decode_lm(stage, meta, 16 * rb + row)), zs, rows.d[rb]);
         } else {
             const float zh = float(zs) * 65536.0f;
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) {
                 float h, l;
-                split_code(decode_end(stage, E0 + rb * step, sh), h, l);
+                split_code(decode_lm(stage, meta, 16 * rb + row), h, l);
                 rows.d[rb] = Mma<true>::mad(h, zh, rows.d[rb]);
                 rows.d[rb] = Mma<true>::mad(l, float(zs), rows.d[rb]);
             }
@@ -695,15 +692,12 @@ __device__ __forceinline__ void right_group(const std::uint32_t* stage, const do
     const int q = lane >> 4, r = lane & 15;
     const std::uint32_t j = g0 + std::uint32_t(a_row<FP32>(r));  // <= 63
     const std::uint32_t meta = std::uint32_t(__shfl(int(rmeta_l), int(j), 64));
-    const std::uint32_t bits = meta >> 16, sh = 32u - bits;
-    std::uint32_t E = (meta & 0xFFFFu) * 32u + (16u * std::uint32_t(q) + 1u) * bits;
     const double* xcol = xs + 16 * q * KB + (r & (KB - 1));
     if constexpr (!WIDE) {
         typename Mm::acc d = Mm::zero();
 #pragma unroll 4
         for (int m = 0; m < 16; ++m) {
-            const int code = decode_end(stage, E, sh);
-            E += bits;
+            const int code = decode_lm(stage, meta, 16 * q + m);
             d = Mm::mad(Real(code), Real(xcol[m * KB]), d);
         }
 #pragma unroll
@@ -712,8 +706,7 @@ __device__ __forceinline__ void right_group(const std::uint32_t* stage, const do
         f32x4 dh = Mma<true>::zero(), dl = Mma<true>::zero();
 #pragma unroll 4
         for (int m = 0; m < 16; ++m) {
-            const int code = decode_end(stage, E, sh);
-            E += bits;
+            const int code = decode_lm(stage, meta, 16 * q + m);
             float h, l;
             split_code(code, h, l);
             const float xv = float(xcol[m * KB]);
@@ -729,8 +722,8 @@ __device__ __forceinline__ void right_group(const std::uint32_t* stage, const do
 // through the LDS table zt, and the left streams of the same components.
 template <int KB, bool FP32, bool WIDE>
 __device__ __forceinline__ void fused_group(const Item& f, const std::uint32_t* stage,
-                                            const double* xs, real_t<FP32>* zt,
-                                            std::uint32_t rmeta_l, std::uint32_t lmeta_l,
+                                            const std::uint32_t* lstage, const double* xs,
+                                            real_t<FP32>* zt, std::uint32_t rmeta_l,
                                             double scale_l, std::uint32_t g0, int lane,
                                             ItemRows<FP32>& rows) {
     using Real = real_t<FP32>;
@@ -744,7 +737,7 @@ __device__ __forceinline__ void fused_group(const Item& f, const std::uint32_t* 
         if (r < KB) zt[(4 * q + t) * KB + r] = c < f.count ? Real(sc * z[t]) : Real(0);
     }
     wave_lds_sync();
-    left_group<KB, FP32, WIDE>(stage, zt, lmeta_l, g0, lane, rows);
+    left_group<KB, FP32, WIDE>(lstage, zt, rmeta_l, g0, lane, rows);
     wave_lds_sync();  // the next group rewrites zt
 }
 
@@ -827,8 +820,8 @@ __device__ __forceinline__ void lowrank_item(const MemberView& M, const Item& f,
                 for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
             }
         }
-        const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
-        const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
+        std::uint32_t total;
+        const std::uint32_t lmeta_l = side_meta(comp, bits_l, lane, total);
         for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16) {
             if (std::uint32_t(lane) - g0 < 16u) {
 #pragma unroll
@@ -839,20 +832,16 @@ __device__ __forceinline__ void lowrank_item(const MemberView& M, const Item& f,
             wave_lds_sync();  // the next group rewrites zt
         }
     } else {
-        const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
-        const std::uint32_t incl_r = wave_scan(nwr, lane);
-        const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
+        std::uint32_t total;
+        const std::uint32_t rmeta_l = side_meta(comp, bits_l, lane, total);
         if (f.kind == kRight) {
             for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
                 right_rows<KB, FP32, WIDE>(M, f, stage, xs, rmeta_l, g0, lane, p0, kk);
             return;
         }
-        const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
-        const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
-        const std::uint32_t lmeta_l =
-            pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
+        const std::uint32_t* lstage = stage + side_words(total);
         for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
-            fused_group<KB, FP32, WIDE>(f, stage, xs, zt, rmeta_l, lmeta_l, scale_l, g0, lane, rows);
+            fused_group<KB, FP32, WIDE>(f, stage, lstage, xs, zt, rmeta_l, scale_l, g0, lane, rows);
     }
     add_item_rows<KB, FP32>(rows, xs, lane, acc);
 }
@@ -904,8 +893,8 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
         Real zs_l[KB];
 #pragma unroll
         for (int p = 0; p < KB; ++p) zs_l[p] = Real(scale_l * z[p]);
-        const std::uint32_t nw = comp ? stream_words(R, bits_l) : 0u;
-        const std::uint32_t lmeta_l = pack_meta(comp, wave_scan(nw, lane) - nw, bits_l);
+        std::uint32_t total;
+        const std::uint32_t lmeta_l = side_meta(comp, bits_l, lane, total);
         valu_left_rows<KB, FP32>(stage, lmeta_l, wide_mask, 0, f.count, lane,
                                  [&](std::uint32_t j, int u, int p) { return lane_value(zs_l[p], j + u); },
                                  acc, blk);
@@ -914,22 +903,20 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
         Real xl[KB];
 #pragma unroll
         for (int p = 0; p < KB; ++p) xl[p] = Real(xs[lane * KB + p]);  // zero past the columns
-        const std::uint32_t nwr = comp ? stream_words(f.cols, bits_l) : 0u;
-        const std::uint32_t incl_r = wave_scan(nwr, lane);
-        const std::uint32_t rmeta_l = pack_meta(comp, incl_r - nwr, bits_l);
-        const std::uint32_t rtotal = __builtin_amdgcn_readlane(incl_r, 63);
-        const std::uint32_t nwl = comp ? stream_words(R, bits_l) : 0u;
-        const std::uint32_t lmeta_l =
-            pack_meta(comp, rtotal + wave_scan(nwl, lane) - nwl, bits_l);
+        // one packed field per component for both side blocks (the left
+        // block of a kFused item follows the right one)
+        std::uint32_t total;
+        const std::uint32_t rmeta_l = side_meta(comp, bits_l, lane, total);
+        const std::uint32_t* lstage = stage + side_words(total);
         // blocks of 8 components, the last of 8 or 4 (the widest components
         // come first, so in FP32 mode only the first blocks usually take the
         // split path)
         std::uint32_t j0 = 0;
         for (; j0 + 4 < f.count; j0 += 8)
-            valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, wide_mask, scale_l, j0,
+            valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
                                               lane, p0, kk, xl, acc, blk);
         if (j0 < f.count)
-            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lmeta_l, wide_mask, scale_l, j0,
+            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
                                               lane, p0, kk, xl, acc, blk);
     }
     if (FP32) {
@@ -1222,6 +1209,7 @@ __global__ __launch_bounds__(256) void reconstruct_tiles(
     const std::int32_t* __restrict__ perm, double* __restrict__ out, std::int64_t ld) {
     __shared__ double L[kTile][kRecComps + 1], Rt[kTile][kRecComps + 1];
     __shared__ std::uint64_t off[2][kRecComps];
+    __shared__ std::uint32_t obit[2][kRecComps];
     const RecTask t = tasks[blockIdx.x];
     const int tid = int(threadIdx.x);
     const int i = tid >> 2, kq = tid & 3;
@@ -1246,20 +1234,24 @@ __global__ __launch_bounds__(256) void reconstruct_tiles(
             const std::uint32_t wn = min(std::uint32_t(kRecComps), t.rank - w0);
             __syncthreads();
             if (tid < 2 * kRecComps) {
-                // first word of component w0 + jj's stream in this tile's chunk
+                // side block holding component w0 + jj in this tile's chunks,
+                // and the component's bit offset in its lanes
                 const int side = tid / kRecComps, jj = tid % kRecComps;
                 const std::uint32_t j = w0 + std::uint32_t(jj);
                 const ChunkRef* c = chunks + (side ? t.rchunk : t.lchunk);
                 const std::uint32_t n = side ? t.nr : t.nl;
                 std::uint64_t w = 0;
+                std::uint32_t o = 0;
                 for (std::uint32_t q = 0; q < n; ++q) {
                     if (j >= c[q].j0 && j < c[q].j0 + c[q].jn) {
                         w = c[q].word;
+                        o = 0;
                         for (std::uint32_t jp = c[q].j0; jp < j; ++jp)
-                            w += stream_words(c[q].count, std::uint32_t(widths[t.comp_base + jp]));
+                            o += std::uint32_t(widths[t.comp_base + jp]);
                     }
                 }
                 off[side][jj] = w;
+                obit[side][jj] = o;
             }
             __syncthreads();
             for (int p = tid; p < kTile * kRecComps; p += 256) {
@@ -1268,10 +1260,12 @@ __global__ __launch_bounds__(256) void reconstruct_tiles(
                     const std::uint32_t c = t.comp_base + w0 + std::uint32_t(jj);
                     const std::uint32_t bits = std::uint32_t(widths[c]);
                     if (row < rows)
-                        L[row][jj] = __dmul_rn(double(decode_code(words + off[0][jj], std::uint32_t(row), bits)),
+                        L[row][jj] = __dmul_rn(double(decode_side_code(words + off[0][jj], std::uint32_t(row),
+                                                                       obit[0][jj], bits)),
                                                lscale[c]);
                     if (row < cols)
-                        Rt[row][jj] = __dmul_rn(double(decode_code(words + off[1][jj], std::uint32_t(row), bits)),
+                        Rt[row][jj] = __dmul_rn(double(decode_side_code(words + off[1][jj], std::uint32_t(row),
+                                                                        obit[1][jj], bits)),
                                                 rscale[c]);
                 }
             }
@@ -1303,12 +1297,12 @@ __global__ void decode_side(const std::uint32_t* __restrict__ words,
                             const std::int32_t* __restrict__ widths, std::uint32_t rows,
                             std::int32_t* __restrict__ out) {
     const ChunkRef ch = chunks[blockIdx.x];
-    std::uint64_t off = ch.word;
+    std::uint32_t o = 0;
     for (std::uint32_t j = ch.j0; j < ch.j0 + ch.jn; ++j) {
         const std::uint32_t bits = widths[j];
         for (std::uint32_t i = threadIdx.x; i < ch.count; i += blockDim.x)
-            out[std::uint64_t(j) * rows + ch.first + i] = decode_code(words + off, i, bits);
-        off += stream_words(ch.count, bits);
+            out[std::uint64_t(j) * rows + ch.first + i] = decode_side_code(words + ch.word, i, o, bits);
+        o += bits;
     }
 }
 
@@ -1480,25 +1474,6 @@ void split_runs(const std::vector<std::uint64_t>& cost, std::uint32_t parts, std
 
 }  // namespace
 
-void append_codes(const std::uint32_t* stream, std::uint64_t stream_words_count,
-                  std::uint64_t first, std::uint64_t count, int bits,
-                  std::vector<std::uint32_t>& out) {
-    const std::uint64_t bit0 = first * std::uint64_t(bits);
-    const std::uint64_t nbits = count * std::uint64_t(bits);
-    const std::uint64_t nw = (nbits + 31) / 32;
-    for (std::uint64_t o = 0; o < nw; ++o) {
-        const std::uint64_t src = bit0 + 32 * o;
-        const std::uint64_t w = src >> 5;
-        const unsigned sh = unsigned(src & 31);
-        std::uint64_t pair = stream[w];
-        if (sh && w + 1 < stream_words_count) pair |= std::uint64_t(stream[w + 1]) << 32;
-        std::uint32_t v = std::uint32_t(pair >> sh);
-        const std::uint64_t left = nbits - 32 * o;
-        if (left < 32) v &= (std::uint32_t{1} << left) - 1u;
-        out.push_back(v);
-    }
-}
-
 int device_count() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -1624,28 +1599,55 @@ Plan build_plan(const qh_store_desc& d) {
         auto& lch = plan.left_chunks.back();
         auto& rch = plan.right_chunks.back();
         const int* wd = d.widths[b];
-        // Components [j0, j) whose streams (per component `per(j)` words) fit
-        // one item with at most `cap` components.
-        auto group_end = [&](std::uint64_t j0, std::uint64_t cap, auto per) {
-            std::uint64_t j = j0, w = 0;
+        // Components [j0, j) of one item with at most `cap` components whose
+        // side blocks fit the item: `sides` blocks of 64 lanes x
+        // ceil(sum of widths / 32) words.
+        auto group_end = [&](std::uint64_t j0, std::uint64_t cap, std::uint64_t sides) {
+            std::uint64_t j = j0, bits = 0;
             while (j < r && j - j0 < cap) {
-                const std::uint64_t add = per(j);
-                if ((w + add + 3) / 4 > std::uint64_t(kItemQuads)) break;
-                w += add;
+                const std::uint64_t nb = bits + std::uint64_t(wd[j]);
+                if (sides * 64 * ((nb + 31) / 32) > 4 * std::uint64_t(kItemQuads)) break;
+                bits = nb;
                 ++j;
             }
-            if (j == j0) throw std::logic_error("device plan: component stream exceeds an item");
+            if (j == j0) throw std::logic_error("device plan: component codes exceed an item");
             return j;
+        };
+        // Side block of components [j0, j1) of one factor over its rows (or
+        // columns) [a, a + n) of the block, n <= 64: lane L holds the codes of
+        // row a + L, one after another (component j at bit offset sum of the
+        // widths before it), word w of lane L at word 64 w + L of the block.
+        auto append_side = [&](const std::uint32_t* const* side_words,
+                               const std::vector<std::uint64_t>& off,
+                               const std::vector<std::uint64_t>& cnt, std::uint64_t j0,
+                               std::uint64_t j1, std::uint64_t a, std::uint64_t n) {
+            std::uint64_t total = 0;
+            for (std::uint64_t j = j0; j < j1; ++j) total += std::uint64_t(wd[j]);
+            const std::uint64_t nw = (total + 31) / 32, base = words.size();
+            words.resize(base + 64 * nw, 0u);
+            for (std::uint64_t L = 0; L < n; ++L) {
+                std::uint64_t o = 0;
+                for (std::uint64_t j = j0; j < j1; ++j) {
+                    const unsigned bits = unsigned(wd[j]);
+                    const std::uint32_t* st = side_words[b] + off[j];
+                    const std::uint64_t src = (a + L) * bits, w = src >> 5;
+                    const unsigned sh = unsigned(src & 31);
+                    std::uint64_t pair = st[w];
+                    if (sh + bits > 32 && w + 1 < cnt[j]) pair |= std::uint64_t(st[w + 1]) << 32;
+                    const std::uint64_t v = (pair >> sh) & ((std::uint64_t{1} << bits) - 1);
+                    const std::uint64_t k = o >> 5, s = o & 31;
+                    words[base + 64 * k + L] |= std::uint32_t(v << s);
+                    if (s + bits > 32) words[base + 64 * (k + 1) + L] |= std::uint32_t(v >> (32 - s));
+                    o += bits;
+                }
+            }
         };
 
         if (rt1 - rt0 == 1 && ct1 - ct0 == 1) {
             // One row tile and one column tile: both factors in one item.
             const std::uint64_t R = m, Cn = nc;
             for (std::uint64_t j0 = 0; j0 < r;) {
-                const std::uint64_t j1 = group_end(j0, kItemComponents, [&](std::uint64_t j) {
-                    return std::uint64_t(stream_words(std::uint32_t(Cn), wd[j]) +
-                                         stream_words(std::uint32_t(R), wd[j]));
-                });
+                const std::uint64_t j1 = group_end(j0, kItemComponents, 2);
                 ItemDev it{};
                 it.quad = begin_item();
                 it.kind = kFused;
@@ -1655,12 +1657,10 @@ Plan build_plan(const qh_store_desc& d) {
                 it.cols = std::uint32_t(Cn);
                 rch.push_back({0u, std::uint32_t(Cn), std::uint32_t(j0), std::uint32_t(j1 - j0),
                                words.size()});
-                for (std::uint64_t j = j0; j < j1; ++j)
-                    append_codes(d.rwords[b] + roff[j], rcnt[j], 0, Cn, wd[j], words);
+                append_side(d.rwords, roff, rcnt, j0, j1, 0, Cn);
                 lch.push_back({0u, std::uint32_t(R), std::uint32_t(j0), std::uint32_t(j1 - j0),
                                words.size()});
-                for (std::uint64_t j = j0; j < j1; ++j)
-                    append_codes(d.lwords[b] + loff[j], lcnt[j], 0, R, wd[j], words);
+                append_side(d.lwords, loff, lcnt, j0, j1, 0, R);
                 finish_item(it);
                 pass1[rt0].push_back(it);
                 j0 = j1;
@@ -1678,9 +1678,7 @@ Plan build_plan(const qh_store_desc& d) {
                     const std::uint64_t a = col_tiles[t].first - c0;
                     const std::uint64_t Cn = col_tiles[t].second - col_tiles[t].first;
                     for (std::uint64_t j0 = 0; j0 < r;) {
-                        const std::uint64_t j1 = group_end(j0, kItemComponents, [&](std::uint64_t j) {
-                            return std::uint64_t(stream_words(std::uint32_t(Cn), wd[j]));
-                        });
+                        const std::uint64_t j1 = group_end(j0, kItemComponents, 1);
                         ItemDev it{};
                         it.quad = begin_item();
                         it.kind = kRight;
@@ -1691,8 +1689,7 @@ Plan build_plan(const qh_store_desc& d) {
                         it.zrow = std::uint32_t(zbase + (t - v0) * r + j0);
                         rch.push_back({std::uint32_t(a), std::uint32_t(Cn), std::uint32_t(j0),
                                        std::uint32_t(j1 - j0), words.size()});
-                        for (std::uint64_t j = j0; j < j1; ++j)
-                            append_codes(d.rwords[b] + roff[j], rcnt[j], a, Cn, wd[j], words);
+                        append_side(d.rwords, roff, rcnt, j0, j1, a, Cn);
                         finish_item(it);
                         right_items.push_back(it);
                         j0 = j1;
@@ -1704,10 +1701,8 @@ Plan build_plan(const qh_store_desc& d) {
                     const std::uint64_t a = row_tiles[t].first - r0;
                     const std::uint64_t R = row_tiles[t].second - row_tiles[t].first;
                     for (std::uint64_t j0 = 0; j0 < r;) {
-                        const std::uint64_t j1 = group_end(j0, std::max<std::uint64_t>(1, 64 / nseg),
-                                                           [&](std::uint64_t j) {
-                            return std::uint64_t(stream_words(std::uint32_t(R), wd[j]));
-                        });
+                        const std::uint64_t j1 =
+                            group_end(j0, std::max<std::uint64_t>(1, 64 / nseg), 1);
                         ItemDev it{};
                         it.quad = begin_item();
                         it.kind = kLeft;
@@ -1719,8 +1714,7 @@ Plan build_plan(const qh_store_desc& d) {
                         if (v0 == ct0)
                             lch.push_back({std::uint32_t(a), std::uint32_t(R), std::uint32_t(j0),
                                            std::uint32_t(j1 - j0), words.size()});
-                        for (std::uint64_t j = j0; j < j1; ++j)
-                            append_codes(d.lwords[b] + loff[j], lcnt[j], a, R, wd[j], words);
+                        append_side(d.lwords, loff, lcnt, j0, j1, a, R);
                         finish_item(it);
                         pass2[t].push_back(it);
                         j0 = j1;
@@ -1737,7 +1731,7 @@ Plan build_plan(const qh_store_desc& d) {
         lr_flops += 2 * r * (m + nc);
     }
     plan.stored_words = words.size();
-    words.resize(words.size() + 8, 0u);  // one readable word past the last item
+    words.resize(words.size() + 72, 0u);  // one readable row of 64 words past the last item
 
     // Reconstruction: one task per (block, row tile, column tile).
     for (std::uint64_t b = 0; b < d.nblocks; ++b) {
@@ -1947,12 +1941,17 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
         check(w < 4 * std::uint64_t(kItemQuads + 1), "stage read out of range");
         return at(pl.words, 4 * std::uint64_t(it.quad) + w);
     };
-    auto code_at = [&](const ItemDev& it, std::uint64_t off, std::uint32_t idx, std::uint32_t bits) {
-        const std::uint64_t pos = std::uint64_t(idx) * bits;
-        check(32 * off + pos + bits <= 128 * std::uint64_t(it.quads), "code outside the item data");
-        const std::uint64_t w = off + (pos >> 5);
-        const std::uint64_t pair = std::uint64_t(word(it, w)) | (std::uint64_t(word(it, w + 1)) << 32);
-        const std::uint64_t raw = (pair >> (pos & 31u)) & ((std::uint64_t{1} << bits) - 1);
+    // Code of lane `lane` at bit offset `o` of the side block starting at
+    // word `side` of the item (word w of the lane at 64 w + lane).
+    auto code_at = [&](const ItemDev& it, std::uint64_t side, std::uint64_t o, std::uint32_t lane,
+                       std::uint32_t bits) {
+        check(lane < 64, "lane out of range");
+        const std::uint64_t k = o >> 5, sh = o & 31;
+        const std::uint64_t last = side + 64 * ((o + bits - 1) >> 5) + lane;
+        check(last < 4 * std::uint64_t(it.quads), "code outside the item data");
+        std::uint64_t pair = word(it, side + 64 * k + lane);
+        if (sh + bits > 32) pair |= std::uint64_t(word(it, side + 64 * (k + 1) + lane)) << 32;
+        const std::uint64_t raw = (pair >> sh) & ((std::uint64_t{1} << bits) - 1);
         const std::uint64_t sign = std::uint64_t{1} << (bits - 1);
         return double(std::int64_t(raw ^ sign) - std::int64_t(sign));
     };
@@ -1964,34 +1963,41 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
     auto xval = [&](std::uint32_t col, std::uint64_t p) {
         return x[std::uint64_t(at(pl.perm, col)) * K + p];
     };
-    // Right-factor sums of components [0, count) of an item whose streams
-    // start at word 0: z[g][p].
+    // Words of a side block of the item's components (64 lanes).
+    auto side_words = [&](const ItemDev& it) {
+        std::uint64_t total = 0;
+        for (std::uint32_t g = 0; g < it.count; ++g) total += at(pl.widths, it.comp0 + g);
+        return 64 * ((total + 31) / 32);
+    };
+    // Right-factor sums of components [0, count) of an item whose right side
+    // block starts at word 0: z[g][p].  Returns the start of the next block.
     auto right_sums = [&](const ItemDev& it, std::vector<double>& z) {
         check(it.cols >= 1 && it.cols <= std::uint32_t(kTile), "item columns");
         z.assign(std::uint64_t(it.count) * K, 0.0);
-        std::uint64_t off = 0;
+        std::uint64_t o = 0;
         for (std::uint32_t g = 0; g < it.count; ++g) {
             const std::uint32_t bits = at(pl.widths, it.comp0 + g);
             for (std::uint64_t p = 0; p < K; ++p) {
                 double s = 0.0;
                 for (std::uint32_t c = 0; c < it.cols; ++c)
-                    s += code_at(it, off, c, bits) * xval(it.col + c, p);
+                    s += code_at(it, 0, o, c, bits) * xval(it.col + c, p);
                 z[g * K + p] = s;
             }
-            off += stream_words(it.cols, bits);
+            o += bits;
         }
-        return off;
+        return side_words(it);
     };
     // Left-factor rows: acc[i][p] += sum_g code(i, g) * cscale_g * z[g][p].
-    auto left_rows = [&](const ItemDev& it, std::uint64_t off, std::uint32_t R,
+    auto left_rows = [&](const ItemDev& it, std::uint64_t side, std::uint32_t R,
                          const std::vector<double>& z, std::vector<double>& acc) {
+        std::uint64_t o = 0;
         for (std::uint32_t g = 0; g < it.count; ++g) {
             const std::uint32_t bits = at(pl.widths, it.comp0 + g);
             const double sc = at(pl.cscale, it.comp0 + g);
             for (std::uint32_t i = 0; i < R; ++i)
                 for (std::uint64_t p = 0; p < K; ++p)
-                    acc[i * K + p] += code_at(it, off, i, bits) * (sc * z[g * K + p]);
-            off += stream_words(R, bits);
+                    acc[i * K + p] += code_at(it, side, o, i, bits) * (sc * z[g * K + p]);
+            o += bits;
         }
     };
     std::vector<double> z;

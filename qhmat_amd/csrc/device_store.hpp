@@ -249,9 +249,4 @@ void profile_begin(DeviceStore& s, std::uint64_t max_applies);
 void set_rhs_block(DeviceStore& s, std::int64_t k, int kb);
 qh_profile profile_end(DeviceStore& s);
 
-// Appends codes [first, first+count) of one component stream of width `bits`
-// to `out` as a fresh word-aligned stream (host side of the upload).
-void append_codes(const std::uint32_t* stream, std::uint64_t stream_words, std::uint64_t first,
-                  std::uint64_t count, int bits, std::vector<std::uint32_t>& out);
-
 }  // namespace qhamd
