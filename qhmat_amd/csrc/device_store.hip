@@ -74,14 +74,26 @@ void require(bool ok, const char* message) {
 // ---------------------------------------------------------------------------
 // Device helpers
 
-// Code of lane `lane` at bit offset `o` of a side block at `side` (word w of
-// a lane at 64 w + lane; global memory).  The next row of the block is read
-// only when the code straddles into it.
+// Slot of lane L in word row w of a side block.  Right-factor blocks are
+// read by lanes that sit on different rows at the same columns (a lane per
+// component), so their rows are swizzled: lane L at L xor 4 (w mod 16),
+// which keeps every aligned group of four columns together and puts the
+// same columns of different rows on different banks.  Left-factor blocks
+// are read a whole row at a time and keep L.
+__host__ __device__ __forceinline__ std::uint32_t lane_slot(bool swz, std::uint64_t w,
+                                                            std::uint64_t L) {
+    return std::uint32_t(swz ? (L ^ (4u * (w & 15u))) : L);
+}
+
+// Code of lane `lane` at bit offset `o` of a side block at `side` (global
+// memory).  The next row of the block is read only when the code straddles
+// into it.
 __device__ __forceinline__ int decode_side_code(const std::uint32_t* side, std::uint32_t lane,
-                                                std::uint32_t o, std::uint32_t bits) {
+                                                std::uint32_t o, std::uint32_t bits, bool swz) {
     const std::uint32_t k = o >> 5, sh = o & 31u;
-    const std::uint32_t lo = side[64u * k + lane];
-    const std::uint32_t hi = sh + bits > 32u ? side[64u * (k + 1u) + lane] : 0u;
+    const std::uint32_t lo = side[64u * k + lane_slot(swz, k, lane)];
+    const std::uint32_t hi =
+        sh + bits > 32u ? side[64u * (k + 1u) + lane_slot(swz, k + 1u, lane)] : 0u;
     const std::uint32_t raw = __builtin_amdgcn_alignbit(hi, lo, sh);
     return bits == 32 ? static_cast<int>(raw) : __builtin_amdgcn_sbfe(raw, 0u, bits);
 }
@@ -128,7 +140,11 @@ constexpr int kLdsHead = 256;  // readable bytes before the stage (decode_lm)
 template <int KB> constexpr int wave_lds_meta() {
     return kLdsHead + kStageBytes + kTile * KB * 8 + 16 * KB * 8;
 }
-template <int KB> constexpr int wave_lds_bytes() { return wave_lds_meta<KB>() + 64 * 12; }
+// Narrow passes in FP32 mode also keep the x rows in FP32 (lanes = columns).
+template <int KB> constexpr int wave_lds_xf() { return wave_lds_meta<KB>() + 64 * 12; }
+template <int KB> constexpr int wave_lds_bytes() {
+    return wave_lds_xf<KB>() + (KB < 8 ? 64 * KB * 4 : 0);
+}
 
 // Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
 // items of the group).
@@ -295,6 +311,21 @@ __device__ __forceinline__ int decode_lm(const std::uint32_t* side, std::uint32_
     return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-64], E)) >> (32u - (m & 63u));
 }
 
+// Codes of columns c4 .. c4 + 3 (c4 a multiple of 4) for the component with
+// packed fields m in a staged right side block, whose rows are swizzled
+// (lane_slot): the four columns stay consecutive within each of the two
+// rows read, so each row is one aligned read of four words per lane.
+__device__ __forceinline__ void decode_right4(const std::uint32_t* side, std::uint32_t m, int c4,
+                                              int (&code)[4]) {
+    const int E = int(m >> 6), kk = E >> 5;
+    const std::uint32_t sh = 32u - (m & 63u);
+    const std::uint32_t* hi = side + 64 * kk + (c4 ^ (4 * (kk & 15)));
+    const std::uint32_t* lo = side + 64 * (kk - 1) + (c4 ^ (4 * ((kk - 1) & 15)));
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        code[u] = static_cast<int>(__builtin_amdgcn_alignbit(hi[u], lo[u], std::uint32_t(E))) >> sh;
+}
+
 // A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
 // exact in FP32, and a code of small magnitude stays in l (no cancellation
 // between the halves).
@@ -361,75 +392,30 @@ __device__ __forceinline__ void swap_rows16(double& a, double& b) {
     b = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi[1]) << 32) | lo[1]));
 }
 
-constexpr int kDppRowMirror = 0x140;      // lane i <- lane 15 - i within 16
-constexpr int kDppRowHalfMirror = 0x141;  // lane i <- lane 7 - i within 8
-constexpr int kDppQuadXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
-constexpr int kDppQuadXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppRowRor4 = 0x124;  // lane i <- lane (i - 4) mod 16 within its row of 16
+constexpr int kDppRowRor8 = 0x128;  // lane i <- lane (i - 8) mod 16 within its row of 16
 
-// Sums N (16, 8 or 4) per-lane values over the 64 lanes.  A butterfly that
-// halves the values at each level, every lane keeping one half and taking
-// its partner's share of it: lane halves (partner lane +-32,
-// v_permlane32_swap), 16-lane rows (+-16, v_permlane16_swap), the mirrored
-// lane within 16 and within 8 (DPP row_mirror / row_half_mirror); once one
-// value is left the remaining levels just add, ending with the quad (DPP).
-// Lane L ends with the total of value block_slot<N>(L); every value's total is
+// Sum of a lane's value over the lanes L, L + N, L + 2N, ... (N = 16, 8 or
+// 4 lanes apart, 64 / N of them): rotations within each row of 16 lanes
+// (DPP) for N < 16, then the 16-lane rows (v_permlane16_swap) and the two
+// halves (v_permlane32_swap).  Every lane of a set ends with the same total,
 // formed by the same tree.
-template <int N, class T>
-__device__ __forceinline__ T reduce_block(const T (&v)[N], int lane) {
-    static_assert(N == 16 || N == 8 || N == 4, "block of 4, 8 or 16 values");
-    T w[N / 2];
-#pragma unroll
-    for (int i = 0; i < N / 2; ++i) {
-        T a = v[2 * i], b = v[2 * i + 1];
-        swap_halves32(a, b);  // lanes 0-31: value 2i; lanes 32-63: value 2i+1
-        w[i] = a + b;
-    }
-    T u[N / 4];
-#pragma unroll
-    for (int i = 0; i < N / 4; ++i) {
-        T a = w[2 * i], b = w[2 * i + 1];
-        swap_rows16(a, b);  // even rows: w[2i]; odd rows: w[2i+1]
-        u[i] = a + b;
-    }
-    const bool b3 = lane & 8, b2 = lane & 4;
-    T s;
-    if constexpr (N == 4) {
-        s = u[0] + dpp_move<kDppRowMirror>(u[0]);
-        s += dpp_move<kDppRowHalfMirror>(s);
-    } else if constexpr (N == 8) {
-        const T t = (b3 ? u[1] : u[0]) + dpp_move<kDppRowMirror>(b3 ? u[0] : u[1]);
-        s = t + dpp_move<kDppRowHalfMirror>(t);
-    } else {
-        T t[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            t[i] = (b3 ? u[2 * i + 1] : u[2 * i]) + dpp_move<kDppRowMirror>(b3 ? u[2 * i] : u[2 * i + 1]);
-        s = (b2 ? t[1] : t[0]) + dpp_move<kDppRowHalfMirror>(b2 ? t[0] : t[1]);
-    }
-    s += dpp_move<kDppQuadXor2>(s);
-    s += dpp_move<kDppQuadXor1>(s);
-    return s;
+template <class T> __device__ __forceinline__ T fold_rows16(T v) {
+    T a = v, b = v;
+    swap_rows16(a, b);  // a: rows 0,0,2,2; b: rows 1,1,3,3
+    return a + b;
 }
-
-// Value whose total lane L holds after reduce_block<N>, and the first lane
-// holding value v.
-template <int N>
-__host__ __device__ __forceinline__ int block_slot(int lane) {
-    const int b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
-    return N == 16 ? 8 * b2 + 4 * b3 + 2 * b4 + b5 : N == 8 ? 4 * b3 + 2 * b4 + b5 : 2 * b4 + b5;
+template <class T> __device__ __forceinline__ T fold_halves32(T v) {
+    T a = v, b = v;
+    swap_halves32(a, b);  // a: the low half twice; b: the high half twice
+    return a + b;
 }
-template <int N>
-__host__ __device__ __forceinline__ int block_lane(int v) {
-    const int s5 = v & 1, s4 = (v >> 1) & 1, s3 = (v >> 2) & 1, s2 = (v >> 3) & 1;
-    return N == 16 ? 32 * s5 + 16 * s4 + 8 * s3 + 4 * s2 : N == 8 ? 32 * s5 + 16 * s4 + 8 * s3
-                                                                  : 32 * s5 + 16 * s4;
+template <int N, class T> __device__ __forceinline__ T group_sum(T v) {
+    static_assert(N == 16 || N == 8 || N == 4, "groups 16, 8 or 4 lanes apart");
+    if constexpr (N == 4) v += dpp_move<kDppRowRor4>(v);
+    if constexpr (N <= 8) v += dpp_move<kDppRowRor8>(v);
+    return fold_halves32(fold_rows16(v));
 }
-// Lanes that write the totals of a block: one per value.
-template <int N>
-__host__ __device__ __forceinline__ bool block_writer(int lane) {
-    return (lane & (N == 16 ? 3 : N == 8 ? 7 : 15)) == 0;
-}
-
 
 // Row contributions of four left-factor components (lanes are rows); zs[u]
 // is component u's scaled input, zero for padding components.  In FP32 mode
@@ -498,49 +484,66 @@ __device__ __forceinline__ void valu_left_rows(const std::uint32_t* stage, std::
 }
 
 // Right-factor sums of a block of N components [j0, j0 + N) of an item
-// (fewer real ones at the end: the rest decode the dummy stream and are
-// never used): lanes are the item's columns, x rows in xl (zero past the
-// last column).  tot[p] in lane L is the sum for component
-// j0 + block_slot<N>(L) and right-hand side p.  In FP32 mode, when a real
-// component of the block is wider than 24 bits, every code of the block is
-// split (split_code) and the halves are summed apart and joined in FP64.
-// The blocks depend only on the item's component count, so a component's
-// rounding is the same for every right-hand-side count.
+// (N = 16, 8 or 4; fewer real ones at the end: the rest decode the dummy
+// field and are never used).  Lane L works on component j0 + L mod N and on
+// the N columns starting at N (L / N): it sums code x x over them in order
+// (x rows from LDS, zero past the item's columns), and group_sum adds the
+// 64 / N column groups.  tot[p] in every lane of component j is its sum for
+// right-hand side p.  In FP32 mode, when a real component of the block is
+// wider than 24 bits, every code of the block is split (split_code) and the
+// halves are summed apart and joined in FP64.  The blocks depend only on
+// the item's component count, so a component's rounding is the same for
+// every right-hand-side count.
 template <int N, int KB, bool FP32>
-__device__ __forceinline__ void valu_right_block(const std::uint32_t* stage, std::uint32_t rmeta_l,
-                                            std::uint64_t wide_mask, std::uint32_t j0, int lane,
-                                            const real_t<FP32> (&xl)[KB], double (&tot)[KB]) {
+__device__ __forceinline__ void valu_right_block(const std::uint32_t* stage,
+                                                 const real_t<FP32>* xs, std::uint32_t rmeta_l,
+                                                 std::uint64_t wide_mask, std::uint32_t j0,
+                                                 int lane, double (&tot)[KB]) {
     using Real = real_t<FP32>;
-    int code[N];
-    const bool wide = ((wide_mask >> j0) & ((1ull << N) - 1ull)) != 0;
+    const int c0 = (lane / N) * N;
+    const std::uint32_t meta =
+        std::uint32_t(__shfl(int(rmeta_l), int(min(j0 + std::uint32_t(lane % N), 63u)), 64));
+    const bool wide = FP32 && ((wide_mask >> j0) & ((1ull << N) - 1ull)) != 0;
+    if (!wide) {
+        Real sum[KB];
 #pragma unroll
-    for (int g = 0; g < N; ++g) {
-        const std::uint32_t meta = __builtin_amdgcn_readlane(rmeta_l, min(j0 + g, 63u));
-        code[g] = decode_lm(stage, meta, lane);
-    }
-    if (!FP32 || !wide) {
+        for (int p = 0; p < KB; ++p) sum[p] = Real(0);
+#pragma unroll 1
+        for (int t = 0; t < N / 4; ++t) {
+            int code[4];
+            decode_right4(stage, meta, c0 + 4 * t, code);
 #pragma unroll
-        for (int p = 0; p < KB; ++p) {
-            Real v[N];
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int g = 0; g < N; ++g) v[g] = Real(code[g]) * xl[p];
-            tot[p] = double(reduce_block<N>(v, lane));
+                for (int p = 0; p < KB; ++p)
+                    sum[p] = fma(Real(code[u]), xs[(c0 + 4 * t + u) * KB + p], sum[p]);
         }
+#pragma unroll
+        for (int p = 0; p < KB; ++p) tot[p] = double(group_sum<N>(sum[p]));
         return;
     }
-    float h[N], l[N];
+    float sh[KB], sl[KB];
 #pragma unroll
-    for (int g = 0; g < N; ++g) split_code(code[g], h[g], l[g]);
+    for (int p = 0; p < KB; ++p) sh[p] = sl[p] = 0.0f;
+#pragma unroll 1
+    for (int t = 0; t < N / 4; ++t) {
+        int code[4];
+        decode_right4(stage, meta, c0 + 4 * t, code);
 #pragma unroll
-    for (int p = 0; p < KB; ++p) {
-        float vh[N], vl[N];
+        for (int u = 0; u < 4; ++u) {
+            float h, l;
+            split_code(code[u], h, l);
 #pragma unroll
-        for (int g = 0; g < N; ++g) {
-            vh[g] = h[g] * float(xl[p]);
-            vl[g] = l[g] * float(xl[p]);
+            for (int p = 0; p < KB; ++p) {
+                const float xv = float(xs[(c0 + 4 * t + u) * KB + p]);  // FP32 mode: already FP32
+                sh[p] = fmaf(h, xv, sh[p]);
+                sl[p] = fmaf(l, xv, sl[p]);
+            }
         }
-        tot[p] = double(reduce_block<N>(vh, lane)) * 65536.0 + double(reduce_block<N>(vl, lane));
     }
+#pragma unroll
+    for (int p = 0; p < KB; ++p)
+        tot[p] = double(group_sum<N>(sh[p])) * 65536.0 + double(group_sum<N>(sl[p]));
 }
 
 // One block of N components of a kRight or kFused item: right-factor sums
@@ -552,14 +555,14 @@ __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const 
                                                 const std::uint32_t* stage, std::uint32_t rmeta_l,
                                                 const std::uint32_t* lstage, std::uint64_t wide_mask,
                                                 double scale_l, std::uint32_t j0, int lane, int p0,
-                                                int kk, const real_t<FP32> (&xl)[KB],
+                                                int kk, const real_t<FP32>* xs,
                                                 double (&acc)[KB], float (&blk)[KB]) {
     using Real = real_t<FP32>;
     double tot[KB];
-    valu_right_block<N, KB, FP32>(stage, rmeta_l, wide_mask, j0, lane, xl, tot);
-    const std::uint32_t jl = j0 + std::uint32_t(block_slot<N>(lane));
+    valu_right_block<N, KB, FP32>(stage, xs, rmeta_l, wide_mask, j0, lane, tot);
+    const std::uint32_t jl = j0 + std::uint32_t(lane % N);
     if (f.kind == kRight) {
-        if (block_writer<N>(lane) && jl < f.count) {
+        if (lane < N && jl < f.count) {
             double* out = M.zp + (std::uint64_t(f.zrow) + jl) * std::uint64_t(M.k) + p0;
 #pragma unroll
             for (int p = 0; p < KB; ++p)
@@ -573,7 +576,7 @@ __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const 
     for (int p = 0; p < KB; ++p) zs_lane[p] = jl < f.count ? Real(sc * tot[p]) : Real(0);
     valu_left_rows<KB, FP32>(lstage, rmeta_l, wide_mask, j0, min(j0 + N, f.count), lane,
                         [&](std::uint32_t j, int u, int p) {
-                            return lane_value(zs_lane[p], block_lane<N>(int(j + u - j0)));
+                            return lane_value(zs_lane[p], j + u - j0);
                         },
                         acc, blk);
 }
@@ -695,23 +698,29 @@ __device__ __forceinline__ void right_group(const std::uint32_t* stage, const do
     const double* xcol = xs + 16 * q * KB + (r & (KB - 1));
     if constexpr (!WIDE) {
         typename Mm::acc d = Mm::zero();
-#pragma unroll 4
-        for (int m = 0; m < 16; ++m) {
-            const int code = decode_lm(stage, meta, 16 * q + m);
-            d = Mm::mad(Real(code), Real(xcol[m * KB]), d);
+#pragma unroll 2
+        for (int t = 0; t < 4; ++t) {
+            int code[4];
+            decode_right4(stage, meta, 16 * q + 4 * t, code);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d = Mm::mad(Real(code[u]), Real(xcol[(4 * t + u) * KB]), d);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) z[t] = double(d[t]);
     } else {
         f32x4 dh = Mma<true>::zero(), dl = Mma<true>::zero();
-#pragma unroll 4
-        for (int m = 0; m < 16; ++m) {
-            const int code = decode_lm(stage, meta, 16 * q + m);
-            float h, l;
-            split_code(code, h, l);
-            const float xv = float(xcol[m * KB]);
-            dh = Mma<true>::mad(h, xv, dh);
-            dl = Mma<true>::mad(l, xv, dl);
+#pragma unroll 2
+        for (int t = 0; t < 4; ++t) {
+            int code[4];
+            decode_right4(stage, meta, 16 * q + 4 * t, code);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float h, l;
+                split_code(code[u], h, l);
+                const float xv = float(xcol[(4 * t + u) * KB]);
+                dh = Mma<true>::mad(h, xv, dh);
+                dl = Mma<true>::mad(l, xv, dl);
+            }
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) z[t] = double(dh[t]) * 65536.0 + double(dl[t]);
@@ -899,25 +908,42 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
                                  [&](std::uint32_t j, int u, int p) { return lane_value(zs_l[p], j + u); },
                                  acc, blk);
     } else {
-        // kFused / kRight: lanes = columns for the right factor
-        Real xl[KB];
-#pragma unroll
-        for (int p = 0; p < KB; ++p) xl[p] = Real(xs[lane * KB + p]);  // zero past the columns
-        // one packed field per component for both side blocks (the left
-        // block of a kFused item follows the right one)
+        // kFused / kRight.  One packed field per component for both side
+        // blocks (the left block of a kFused item follows the right one).
         std::uint32_t total;
         const std::uint32_t rmeta_l = side_meta(comp, bits_l, lane, total);
         const std::uint32_t* lstage = stage + side_words(total);
-        // blocks of 8 components, the last of 8 or 4 (the widest components
-        // come first, so in FP32 mode only the first blocks usually take the
-        // split path)
+        // x rows in the arithmetic precision (FP32 mode: converted once)
+        const Real* xr;
+        if constexpr (FP32) {
+            float* xf = reinterpret_cast<float*>(wave_lds + wave_lds_xf<KB>());
+#pragma unroll
+            for (int p = 0; p < KB; ++p) xf[lane * KB + p] = float(xs[lane * KB + p]);
+            wave_lds_sync();
+            xr = xf;
+        } else {
+            xr = xs;
+        }
+        // In FP32 mode the components up to the last one wider than 24 bits
+        // (the widest come first) go in blocks of 4, each deciding the split
+        // for itself; then blocks of 16 while more than 8 remain, then one of
+        // 8 or 4.
         std::uint32_t j0 = 0;
-        for (; j0 + 4 < f.count; j0 += 8)
+        const std::uint32_t wide_end = wide_mask ? 64u - std::uint32_t(__builtin_clzll(wide_mask)) : 0u;
+        for (; j0 < wide_end; j0 += 4)
+            valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
+                                              lane, p0, kk, xr, acc, blk);
+        for (; j0 + 8 < f.count; j0 += 16)
+            valu_right_then_left<16, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
+                                               lane, p0, kk, xr, acc, blk);
+        if (j0 + 4 < f.count) {
             valu_right_then_left<8, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
-                                              lane, p0, kk, xl, acc, blk);
+                                              lane, p0, kk, xr, acc, blk);
+            j0 += 8;
+        }
         if (j0 < f.count)
             valu_right_then_left<4, KB, FP32>(M, f, stage, rmeta_l, lstage, wide_mask, scale_l, j0,
-                                              lane, p0, kk, xl, acc, blk);
+                                              lane, p0, kk, xr, acc, blk);
     }
     if (FP32) {
 #pragma unroll
@@ -1261,11 +1287,11 @@ __global__ __launch_bounds__(256) void reconstruct_tiles(
                     const std::uint32_t bits = std::uint32_t(widths[c]);
                     if (row < rows)
                         L[row][jj] = __dmul_rn(double(decode_side_code(words + off[0][jj], std::uint32_t(row),
-                                                                       obit[0][jj], bits)),
+                                                                       obit[0][jj], bits, false)),
                                                lscale[c]);
                     if (row < cols)
                         Rt[row][jj] = __dmul_rn(double(decode_side_code(words + off[1][jj], std::uint32_t(row),
-                                                                        obit[1][jj], bits)),
+                                                                        obit[1][jj], bits, true)),
                                                 rscale[c]);
                 }
             }
@@ -1295,13 +1321,14 @@ __global__ __launch_bounds__(256) void reconstruct_tiles(
 __global__ void decode_side(const std::uint32_t* __restrict__ words,
                             const ChunkRef* __restrict__ chunks,
                             const std::int32_t* __restrict__ widths, std::uint32_t rows,
-                            std::int32_t* __restrict__ out) {
+                            int swz, std::int32_t* __restrict__ out) {
     const ChunkRef ch = chunks[blockIdx.x];
     std::uint32_t o = 0;
     for (std::uint32_t j = ch.j0; j < ch.j0 + ch.jn; ++j) {
         const std::uint32_t bits = widths[j];
         for (std::uint32_t i = threadIdx.x; i < ch.count; i += blockDim.x)
-            out[std::uint64_t(j) * rows + ch.first + i] = decode_side_code(words + ch.word, i, o, bits);
+            out[std::uint64_t(j) * rows + ch.first + i] =
+                decode_side_code(words + ch.word, i, o, bits, swz != 0);
         o += bits;
     }
 }
@@ -1616,11 +1643,12 @@ Plan build_plan(const qh_store_desc& d) {
         // Side block of components [j0, j1) of one factor over its rows (or
         // columns) [a, a + n) of the block, n <= 64: lane L holds the codes of
         // row a + L, one after another (component j at bit offset sum of the
-        // widths before it), word w of lane L at word 64 w + L of the block.
+        // widths before it), word w of lane L at word 64 w + lane_slot(w, L)
+        // of the block (swizzled for right factors).
         auto append_side = [&](const std::uint32_t* const* side_words,
                                const std::vector<std::uint64_t>& off,
                                const std::vector<std::uint64_t>& cnt, std::uint64_t j0,
-                               std::uint64_t j1, std::uint64_t a, std::uint64_t n) {
+                               std::uint64_t j1, std::uint64_t a, std::uint64_t n, bool swz) {
             std::uint64_t total = 0;
             for (std::uint64_t j = j0; j < j1; ++j) total += std::uint64_t(wd[j]);
             const std::uint64_t nw = (total + 31) / 32, base = words.size();
@@ -1636,8 +1664,9 @@ Plan build_plan(const qh_store_desc& d) {
                     if (sh + bits > 32 && w + 1 < cnt[j]) pair |= std::uint64_t(st[w + 1]) << 32;
                     const std::uint64_t v = (pair >> sh) & ((std::uint64_t{1} << bits) - 1);
                     const std::uint64_t k = o >> 5, s = o & 31;
-                    words[base + 64 * k + L] |= std::uint32_t(v << s);
-                    if (s + bits > 32) words[base + 64 * (k + 1) + L] |= std::uint32_t(v >> (32 - s));
+                    words[base + 64 * k + lane_slot(swz, k, L)] |= std::uint32_t(v << s);
+                    if (s + bits > 32)
+                        words[base + 64 * (k + 1) + lane_slot(swz, k + 1, L)] |= std::uint32_t(v >> (32 - s));
                     o += bits;
                 }
             }
@@ -1657,10 +1686,10 @@ Plan build_plan(const qh_store_desc& d) {
                 it.cols = std::uint32_t(Cn);
                 rch.push_back({0u, std::uint32_t(Cn), std::uint32_t(j0), std::uint32_t(j1 - j0),
                                words.size()});
-                append_side(d.rwords, roff, rcnt, j0, j1, 0, Cn);
+                append_side(d.rwords, roff, rcnt, j0, j1, 0, Cn, true);
                 lch.push_back({0u, std::uint32_t(R), std::uint32_t(j0), std::uint32_t(j1 - j0),
                                words.size()});
-                append_side(d.lwords, loff, lcnt, j0, j1, 0, R);
+                append_side(d.lwords, loff, lcnt, j0, j1, 0, R, false);
                 finish_item(it);
                 pass1[rt0].push_back(it);
                 j0 = j1;
@@ -1689,7 +1718,7 @@ Plan build_plan(const qh_store_desc& d) {
                         it.zrow = std::uint32_t(zbase + (t - v0) * r + j0);
                         rch.push_back({std::uint32_t(a), std::uint32_t(Cn), std::uint32_t(j0),
                                        std::uint32_t(j1 - j0), words.size()});
-                        append_side(d.rwords, roff, rcnt, j0, j1, a, Cn);
+                        append_side(d.rwords, roff, rcnt, j0, j1, a, Cn, true);
                         finish_item(it);
                         right_items.push_back(it);
                         j0 = j1;
@@ -1714,7 +1743,7 @@ Plan build_plan(const qh_store_desc& d) {
                         if (v0 == ct0)
                             lch.push_back({std::uint32_t(a), std::uint32_t(R), std::uint32_t(j0),
                                            std::uint32_t(j1 - j0), words.size()});
-                        append_side(d.lwords, loff, lcnt, j0, j1, a, R);
+                        append_side(d.lwords, loff, lcnt, j0, j1, a, R, false);
                         finish_item(it);
                         pass2[t].push_back(it);
                         j0 = j1;
@@ -1944,13 +1973,14 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
     // Code of lane `lane` at bit offset `o` of the side block starting at
     // word `side` of the item (word w of the lane at 64 w + lane).
     auto code_at = [&](const ItemDev& it, std::uint64_t side, std::uint64_t o, std::uint32_t lane,
-                       std::uint32_t bits) {
+                       std::uint32_t bits, bool swz) {
         check(lane < 64, "lane out of range");
-        const std::uint64_t k = o >> 5, sh = o & 31;
-        const std::uint64_t last = side + 64 * ((o + bits - 1) >> 5) + lane;
+        const std::uint64_t k = o >> 5, sh = o & 31, k1 = (o + bits - 1) >> 5;
+        const std::uint64_t last = side + 64 * k1 + lane_slot(swz, k1, lane);
         check(last < 4 * std::uint64_t(it.quads), "code outside the item data");
-        std::uint64_t pair = word(it, side + 64 * k + lane);
-        if (sh + bits > 32) pair |= std::uint64_t(word(it, side + 64 * (k + 1) + lane)) << 32;
+        std::uint64_t pair = word(it, side + 64 * k + lane_slot(swz, k, lane));
+        if (sh + bits > 32)
+            pair |= std::uint64_t(word(it, side + 64 * (k + 1) + lane_slot(swz, k + 1, lane))) << 32;
         const std::uint64_t raw = (pair >> sh) & ((std::uint64_t{1} << bits) - 1);
         const std::uint64_t sign = std::uint64_t{1} << (bits - 1);
         return double(std::int64_t(raw ^ sign) - std::int64_t(sign));
@@ -1980,7 +2010,7 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
             for (std::uint64_t p = 0; p < K; ++p) {
                 double s = 0.0;
                 for (std::uint32_t c = 0; c < it.cols; ++c)
-                    s += code_at(it, 0, o, c, bits) * xval(it.col + c, p);
+                    s += code_at(it, 0, o, c, bits, true) * xval(it.col + c, p);
                 z[g * K + p] = s;
             }
             o += bits;
@@ -1996,7 +2026,7 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
             const double sc = at(pl.cscale, it.comp0 + g);
             for (std::uint32_t i = 0; i < R; ++i)
                 for (std::uint64_t p = 0; p < K; ++p)
-                    acc[i * K + p] += code_at(it, side, o, i, bits) * (sc * z[g * K + p]);
+                    acc[i * K + p] += code_at(it, side, o, i, bits, false) * (sc * z[g * K + p]);
             o += bits;
         }
     };
@@ -2180,7 +2210,7 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
                              hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(decode_side, dim3(unsigned(chunks.size())), dim3(64), 0, st, s.words,
-                           dev, s.widths + s.comp_base[li], rows, out);
+                           dev, s.widths + s.comp_base[li], rows, side == 0 ? 0 : 1, out);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
