@@ -229,14 +229,16 @@ class Server:
             for reqs, y, cost in work:
                 col = 0
                 for r in reqs:
+                    # statistics first, so that they include a request once its
+                    # result is visible
+                    self.latencies.append(now - r.arrival)
+                    self.completed += 1
+                    self.completed_rhs += r.k
                     if r.finite is not None and not bool(r.finite):
                         r.future.set_exception(ValueError("nonfinite right-hand side"))
                     else:
                         r.future.set_result(y[:, col:col + r.k])
                     col += r.k
-                    self.latencies.append(now - r.arrival)
-                    self.completed += 1
-                    self.completed_rhs += r.k
                 freed += cost
             with self._cv:
                 self._inflight_bytes -= freed
@@ -244,6 +246,13 @@ class Server:
             self._slots.release()
 
     # -- statistics ---------------------------------------------------------------------------
+    def reset_stats(self) -> None:
+        """Forgets the latencies and counts recorded so far (e.g. of a warm-up)."""
+        self.latencies = []
+        self.passes = 0
+        self.completed = 0
+        self.completed_rhs = 0
+
     def latency_ms(self) -> dict:
         lat = np.asarray(self.latencies) * 1e3
         if lat.size == 0:
@@ -257,8 +266,10 @@ def poisson_load(server: Server, names: list[str], orders: dict[str, int], rate:
     """Open-loop Poisson arrivals at `rate` req/s for `duration` seconds.
 
     Requests come in sessions of `session` consecutive requests (a GMRES
-    client's iterations) that target one store with one k.  Returns
-    throughput and latency figures.
+    client's iterations) that target one store with one k.  One request per
+    (store, k) is served first and its statistics dropped, so the figures are
+    of the steady state (first launches of each kernel shape, workspace
+    growth).  Returns throughput and latency figures.
     """
     import torch
 
@@ -270,6 +281,9 @@ def poisson_load(server: Server, names: list[str], orders: dict[str, int], rate:
     xs = {(nm, k): torch.randn(orders[nm], k, dtype=torch.float64, device=f"cuda:{server.device}")
           for nm in names for k in ks}
     torch.cuda.synchronize()
+    for key in xs:
+        server.submit(key[0], xs[key]).result()
+    server.reset_stats()
     futures = []
     # The generator sleeps until each arrival (a busy-wait would hold the
     # interpreter lock from the dispatcher), with a short switch interval so
