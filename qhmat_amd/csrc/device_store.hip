@@ -205,15 +205,17 @@ __device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, std::uint32
         (static_cast<unsigned long long>(v[1]) << 32) | static_cast<unsigned long long>(v[0])));
 }
 
-__device__ __forceinline__ std::int32_t perm_row(const MemberView& M, const Item& f, int lane) {
-    const std::uint32_t n = x_columns(f);
-    const __amdgpu_buffer_rsrc_t r = buffer(M.perm + f.col, n * 4u);
-    return std::int32_t(__builtin_amdgcn_raw_buffer_load_b32(r, std::uint32_t(lane) * 4u, 0, 0));
+// Row sets per tile in the workspace (four workgroups per pass), and x in
+// clustered order after them: xc[i][p] = x[perm[i]][p], written once per
+// apply (permute_tile) so that an item's x rows are one contiguous run.
+constexpr int kRowSets = 2 * kPass1Groups;
+__device__ __forceinline__ double* clustered_x(const MemberView& M) {
+    return M.zp + (M.z_rows + std::uint64_t(kRowSets) * M.order) * std::uint64_t(M.k);
 }
 
 template <int KB>
-__device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::int32_t prow,
-                                      int lane, int p0, int kk, Fetch<KB>& F) {
+__device__ __forceinline__ void fetch(const MemberView& M, const Item& f, int lane, int p0, int kk,
+                                      Fetch<KB>& F) {
     const __amdgpu_buffer_rsrc_t data =
         buffer(M.words + 4ull * f.quad, f.quads * 16u);
 #pragma unroll
@@ -224,8 +226,8 @@ __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::i
                                                   std::uint32_t(lane) * 4u, 0, 0);
     const std::uint32_t nscale = (f.kind == kFused || f.kind == kLeft) ? f.count : 0u;
     F.scale = load_f64(buffer(M.cscale + f.comp0, nscale * 8u), std::uint32_t(lane) * 8u);
-    // x and the workspace: byte offsets from the call's base pointers (the
-    // host keeps both below kOutOfRange bytes)
+    // clustered x and the partial sums: byte offsets in the workspace (the
+    // host keeps it below kOutOfRange bytes)
     const std::uint32_t k = std::uint32_t(M.k);
     std::uint32_t row;
     bool mine;
@@ -237,8 +239,8 @@ __device__ __forceinline__ void fetch(const MemberView& M, const Item& f, std::i
         r = buffer(M.zp, kOutOfRange);
     } else {
         mine = lane < int(x_columns(f));
-        row = std::uint32_t(prow) * std::uint32_t(M.ldx);
-        r = buffer(M.x, kOutOfRange);
+        row = (f.col + std::uint32_t(lane)) * k;
+        r = buffer(clustered_x(M), kOutOfRange);
     }
 #pragma unroll
     for (int p = 0; p < KB; ++p)
@@ -978,8 +980,7 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
 // One wave's items: [b1, e1) followed by [b2, e2).  Row contributions
 // (kDense, kFused, kLeft) are added to acc (lanes = the R rows of the tile);
 // kRight items write their partial sums to the workspace.  Two items are in
-// flight while one is decoded: the fetch buffers alternate, and the
-// permutation rows of an item are requested two steps before its fetch.
+// flight while one is decoded: the fetch buffers alternate.
 template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1, std::uint32_t e1,
                                           std::uint32_t b2, std::uint32_t e2, int p0, int kk,
@@ -1004,31 +1005,23 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
         auto slot = [&](std::uint32_t i) { return i < n ? item_at(r0, r1, i) : none; };
         Item fa = slot(0), fb = slot(1), fc = slot(2), fd = slot(3);
         Fetch<KB> FA, FB;
-        // Permutation rows of the next fetch into buffer A (PA) and into B
-        // (PB), requested two steps before that fetch.
-        std::int32_t PA = perm_row(M, fa, lane), PB = perm_row(M, fb, lane);
-        fetch<KB>(M, fa, PA, lane, p0, kk, FA);
-        PA = perm_row(M, fc, lane);
-        fetch<KB>(M, fb, PB, lane, p0, kk, FB);
-        PB = perm_row(M, fd, lane);
+        fetch<KB>(M, fa, lane, p0, kk, FA);
+        fetch<KB>(M, fb, lane, p0, kk, FB);
 
         // Steady state: buffer A holds item s, buffer B item s+1.  Step s
-        // waits for item s (fetched two steps earlier), fetches item s+2 with
-        // rows requested two steps earlier and requests the rows of item s+4;
-        // neither wait covers item s+1, so two items' data stay in flight
-        // during a decode.
+        // waits for item s (fetched two steps earlier) and fetches item s+2
+        // into the freed buffer, so two items' data stay in flight during a
+        // decode.
         for (std::uint32_t s = 0; s < n; s += 2) {
             {
                 stage_item<KB>(fa, FA, lane, wave_lds);
-                fetch<KB>(M, fc, PA, lane, p0, kk, FA);
-                const Item fe = slot(s + 4);
-                PA = perm_row(M, fe, lane);
+                fetch<KB>(M, fc, lane, p0, kk, FA);
                 wave_lds_sync();
                 compute_item<KB, FP32, PASS>(M, fa, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
                                              R, wave_lds, acc);
                 wave_lds_sync();  // the next item overwrites the stage
                 fa = fc;
-                fc = fe;
+                fc = slot(s + 4);
             }
             // Narrow passes take no exit here: an odd count runs one empty
             // item, whose loads all read zero, so that both steps always run
@@ -1039,15 +1032,13 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
             }
             {
                 stage_item<KB>(fb, FB, lane, wave_lds);
-                fetch<KB>(M, fd, PB, lane, p0, kk, FB);
-                const Item fe = slot(s + 5);
-                PB = perm_row(M, fe, lane);
+                fetch<KB>(M, fd, lane, p0, kk, FB);
                 wave_lds_sync();
                 compute_item<KB, FP32, PASS>(M, fb, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
                                              R, wave_lds, acc);
                 wave_lds_sync();
                 fb = fd;
-                fd = fe;
+                fd = slot(s + 5);
             }
         }
     }
@@ -1097,7 +1088,6 @@ __device__ __forceinline__ void combine_rows(const double (&total)[KB], const do
 // over the pass-1 sets g and the pass-2 sets h: one fixed tree over the 32
 // runs, the same for every right-hand-side count and launch shape.
 constexpr int kGroupWaves = kWaves / kPass1Groups;
-constexpr int kRowSets = 2 * kPass1Groups;
 
 template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void group_rows(const MemberView& M, std::uint32_t group, int p0,
@@ -1173,6 +1163,20 @@ __global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_p
 
 __global__ __launch_bounds__(256) void apply_combine(MemberView M) { combine_tile(M, blockIdx.x); }
 
+// x rows of tile t's indices in clustered order (the tiles cover every index
+// once), one thread per (row, right-hand side).
+__device__ __forceinline__ void permute_tile(const MemberView& M, std::uint32_t t) {
+    const TileDev* __restrict__ tile = M.tiles + t;
+    const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
+    const std::uint32_t k = std::uint32_t(M.k);
+    double* xc = clustered_x(M) + std::uint64_t(tile->row0) * k;
+    for (std::uint32_t e = threadIdx.x; e < R * k; e += blockDim.x) {
+        const std::uint32_t i = e / k, p = e - i * k;
+        xc[e] = M.x[std::int64_t(M.perm[tile->row0 + i]) * M.ldx + p];
+    }
+}
+__global__ __launch_bounds__(256) void apply_permute(MemberView M) { permute_tile(M, blockIdx.x); }
+
 // The member owning global work index `index`: first[m] <= index < first[m+1].
 __device__ __forceinline__ std::uint32_t locate_member(const std::uint32_t* __restrict__ first,
                                                        std::uint32_t count, std::uint32_t index) {
@@ -1208,6 +1212,15 @@ __global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
     group_rows<KB, FP32, PASS>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
+}
+
+__global__ __launch_bounds__(256) void batch_permute(const MemberView* __restrict__ members,
+                                                    const std::uint32_t* __restrict__ first,
+                                                    std::uint32_t count, const double* x,
+                                                    double* y, double* zp) {
+    const std::uint32_t m = locate_member(first, count, blockIdx.x);
+    const MemberView M = bind_member(members, m, x, y, zp);
+    permute_tile(M, blockIdx.x - first[m]);
 }
 
 __global__ __launch_bounds__(256) void batch_combine(const MemberView* __restrict__ members,
@@ -1383,6 +1396,8 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
     if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
+    hipLaunchKernelGGL(apply_permute, dim3(s.n_tiles), dim3(256), 0, stream, M);
+    hip_check(hipGetLastError(), "apply permute launch");
     hipLaunchKernelGGL((apply_pass1<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
@@ -1400,6 +1415,9 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
                         hipStream_t stream) {
     const unsigned passes = unsigned((c.max_k + KB - 1) / KB);
     const std::uint32_t count = std::uint32_t(c.member_count);
+    hipLaunchKernelGGL(batch_permute, dim3(c.total_tiles), dim3(256), 0, stream, c.members,
+                       c.first_tile, count, x, y, zp);
+    hip_check(hipGetLastError(), "batched permute launch");
     hipLaunchKernelGGL((batch_pass<KB, FP32, 1>), dim3(c.total_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 1 launch");
@@ -2120,7 +2138,8 @@ void destroy_device_store(DeviceStore* s) {
 }
 
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k) {
-    return (s.z_rows + kRowSets * s.order) * std::uint64_t(k) * sizeof(double);
+    // partial sums, the row sets, then x in clustered order
+    return (s.z_rows + (kRowSets + 1) * s.order) * std::uint64_t(k) * sizeof(double);
 }
 
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,

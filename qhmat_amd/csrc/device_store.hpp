@@ -22,17 +22,21 @@
 //          [comp0, comp0+count): left-factor streams only.  Reads the
 //          partial sums of the block's column tiles from the workspace.
 //
-// A stream is the reference's packed format (core.cpp:235-267) restricted to
-// the tile's rows or columns: codes LSB-first, b bits each, two's complement.
-// When a tile starts on a multiple of 32 codes its stream words are the
-// reference's words verbatim.
+// Codes form side blocks of 64 lanes (lane L = the tile's row or column L):
+// lane L's codes of the item's components follow one another in its own
+// LSB-first bit stream (the reference's two's-complement b-bit codes,
+// core.cpp:235-267), word w of lane L at word 64 w + slot(w, L) of the block
+// (right-factor blocks swizzled by row, see lane_slot).  A kFused item holds
+// its right block, then its left block.
 //
-// An apply is two launches:
-//  pass 1  four workgroups per row tile, each running four of the tile's 16
-//          item runs: the tile's kDense and kFused items, whose row sums go
-//          to the workspace, and a share of the kRight items;
-//  pass 2  one workgroup per row tile: the tile's kLeft items plus the
-//          pass-1 row sums, written to y through the permutation.
+// An apply is four launches:
+//  permute  x in clustered order into the workspace;
+//  pass 1   four workgroups per row tile, each running four of the tile's 16
+//           item runs: the tile's kDense and kFused items, whose row sums go
+//           to the workspace, and a share of the kRight items;
+//  pass 2   the same shape over the tile's kLeft items;
+//  combine  one workgroup per row tile sums the eight row sets into y
+//           through the permutation.
 #pragma once
 
 #include <cstdint>
@@ -94,7 +98,8 @@ struct MemberView {
     const TileDev* tiles;
     const double* x;
     double* y;
-    double* zp;     // workspace: z_rows x k partial sums, then 2 kPass1Groups sets of
+    double* zp;     // workspace: z_rows x k partial sums, x in clustered order after the
+                    // row sets; row sets: 2 kPass1Groups sets of
                     // order x k row sums (four per pass)
     std::int64_t ldx, ldy;
     std::int32_t k;
@@ -143,7 +148,7 @@ struct DeviceStore {
     // sizes
     std::uint64_t n_words = 0, n_comp = 0;
     std::uint32_t n_tiles = 0, n_items = 0;
-    std::uint64_t z_rows = 0;  // workspace = (z_rows + 2 kPass1Groups order) k doubles
+    std::uint64_t z_rows = 0;  // workspace = (z_rows + (2 kPass1Groups + 1) order) k doubles
     // host mirrors needed by entry points
     std::vector<std::int64_t> block_to_lowrank;  // insertion index -> low-rank index or -1
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;  // per low-rank block
