@@ -169,7 +169,8 @@ qh_status qh_graph_destroy(qh_graph *graph);
  * `max_applies` calls on this store (the events are created here, not in the
  * timed calls); qh_profile_end waits for the recorded events, returns the
  * totals and disarms.  pass_ms[0] = pass 1 (right factors, diagonal and fused
- * blocks), pass_ms[1] = pass 2 (left factors) and the combine launch. */
+ * blocks), pass_ms[1] = pass 2 (left factors) and the combine launch; the
+ * short launch that puts x in clustered order runs before both. */
 typedef struct {
     uint64_t applies;
     double pass_ms[2];

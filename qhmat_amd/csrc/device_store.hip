@@ -1395,9 +1395,9 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
     const unsigned passes = unsigned((k + KB - 1) / KB);
     const MemberView M = member_view(s, x, ldx, k, y, ldy, zp);
     hipEvent_t* ev = profile_slot(s);
-    if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
     hipLaunchKernelGGL(apply_permute, dim3(s.n_tiles), dim3(256), 0, stream, M);
     hip_check(hipGetLastError(), "apply permute launch");
+    if (ev) hip_check(hipEventRecord(ev[0], stream), "hipEventRecord");
     hipLaunchKernelGGL((apply_pass1<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
