@@ -138,6 +138,20 @@ qh_status qh_apply(const qh_store *store, const double *x, int64_t ldx, int64_t 
                    int64_t ldy, int precision, void *stream, void *workspace,
                    uint64_t workspace_bytes);
 
+/* qh_apply without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * workspace is a static device array of qh_noalloc_scratch_bytes() (64 MiB)
+ * in each device's memory, created when the library's code object is loaded
+ * and held until the process ends; its address is looked up once per device
+ * on the first call there.  An apply whose qh_apply_workspace_bytes exceeds
+ * it fails with QH_EINVAL.  Every call on one device shares that device's
+ * array, so calls that overlap on one device (two streams, two threads)
+ * race: serialise them.  Same kernels and results as qh_apply. */
+qh_status qh_apply_noalloc(const qh_store *store, const double *x, int64_t ldx, int64_t k,
+                           double *y, int64_t ldy, int precision, void *stream);
+qh_status qh_noalloc_scratch_bytes(uint64_t *bytes);
+
 /* ---- batches of resident stores (no counterpart in the reference) -------
  * Many stores of uneven order, rank and right-hand-side count applied
  * together: one launch pair per right-hand-side class (k rounded up to

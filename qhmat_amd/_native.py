@@ -108,6 +108,8 @@ _sig("qh_store_destroy", _p)
 _sig("qh_store_get_stats", _p, C.POINTER(StoreStats))
 _sig("qh_apply_workspace_bytes", _p, _i64, C.POINTER(_u64))
 _sig("qh_apply", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p, _p, _u64)
+_sig("qh_apply_noalloc", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p)
+_sig("qh_noalloc_scratch_bytes", C.POINTER(_u64))
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
 _sig("qh_reconstruct", _p, _p, _i64, _p)
 _sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)

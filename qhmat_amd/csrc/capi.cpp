@@ -139,6 +139,21 @@ qh_status qh_apply(const qh_store* store, const double* x, int64_t ldx, int64_t 
     });
 }
 
+qh_status qh_apply_noalloc(const qh_store* store, const double* x, int64_t ldx, int64_t k,
+                           double* y, int64_t ldy, int precision, void* stream) {
+    return guarded([&] {
+        need(store, "store");
+        qhamd::apply_noalloc(*store->dev, x, ldx, k, y, ldy, precision, stream);
+    });
+}
+
+qh_status qh_noalloc_scratch_bytes(uint64_t* bytes) {
+    return guarded([&] {
+        need(bytes, "bytes");
+        *bytes = qhamd::kScratchBytes;
+    });
+}
+
 qh_status qh_batch_create(const qh_store* const* stores, const int64_t* k, uint64_t count,
                           qh_batch** out) {
     return guarded([&] {

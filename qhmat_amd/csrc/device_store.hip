@@ -2177,6 +2177,42 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
     if (owned) hip_check(hipFreeAsync(zp, st), "hipFreeAsync");
 }
 
+// Workspace of qh_apply_noalloc: one array in every device's memory, made
+// when the code object is loaded and held until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: its
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) std::uint8_t scratch[kScratchBytes];
+
+namespace {
+std::mutex scratch_lock;
+std::map<int, void*> scratch_ptr;  // device ordinal -> address of that device's array
+
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> guard(scratch_lock);
+    const auto it = scratch_ptr.find(device);
+    if (it != scratch_ptr.end()) return it->second;
+    DeviceScope scope(device);
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+    scratch_ptr[device] = ptr;
+    return ptr;
+}
+}  // namespace
+
+void apply_noalloc(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k,
+                   double* y, std::int64_t ldy, int precision, void* stream) {
+    require(k > 0 && k <= std::int64_t(std::numeric_limits<int>::max()),
+            "right-hand-side shape mismatch");
+    const std::uint64_t need = apply_workspace_bytes(s, k);
+    if (need > kScratchBytes)
+        throw std::invalid_argument("apply needs " + std::to_string(need) +
+                                    " workspace bytes, more than the static scratch holds (" +
+                                    std::to_string(kScratchBytes) + "); use qh_apply");
+    void* ws = scratch_address(s.device);
+    apply(s, x, ldx, k, y, ldy, precision, stream, ws, kScratchBytes);
+}
+
 void profile_begin(DeviceStore& s, std::uint64_t max_applies) {
     require(max_applies > 0 && max_applies < (std::uint64_t{1} << 20), "invalid profile size");
     DeviceScope scope(s.device);

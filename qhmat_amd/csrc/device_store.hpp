@@ -244,6 +244,13 @@ std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k);
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,
            std::int64_t ldy, int precision, void* stream, void* workspace,
            std::uint64_t workspace_bytes);
+// Bytes of the static device scratch that qh_apply_noalloc uses as its
+// workspace (one array per device, held until the process ends).
+constexpr std::uint64_t kScratchBytes = std::uint64_t{64} << 20;
+// qh_apply with that scratch as the workspace: no allocation of any kind.
+// Rejects (std::invalid_argument) an apply that needs more workspace.
+void apply_noalloc(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k,
+                   double* y, std::int64_t ldy, int precision, void* stream);
 void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int32_t* out,
                   void* stream);
 // Dense n x n FP64 matrix in source order into device memory (row-major,
