@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """H-matvec throughput of the MI355X apply on one GPU.
 
-Workload: 2-D kernel matrix, n=16384 points uniform in the unit square
-(seed 0), A_ij = exp(-|x_i - x_j| / 0.2), leaf 64, tolerance 1e-6 (the
-tightest tolerance the reference constructor accepts for this matrix; 1e-8
-makes every sampled block infeasible), full width menu; right-hand sides
-N(0,1) FP64 (seed 1).  One step = one apply y = H X of the resident store.
+Headline workload (the metric's configuration): 2-D kernel matrix, n=16384
+points uniform in the unit square (seed 0), A_ij = exp(-|x_i - x_j| / 0.2),
+leaf 64, tolerance 1e-6 (the tightest tolerance the reference constructor
+accepts for this matrix; 1e-8 makes every sampled block infeasible), full
+width menu; right-hand sides N(0,1) FP64 (seed 1).  One step = one apply
+y = H X of the resident store.  Copies of the store are rotated (--copies,
+default 4: ~370 MB > the 256 MB last-level cache) so the timed figure is
+HBM-bound, not cache-resident.
 
-Copies of the store are rotated (--copies, default 4: ~370 MB > the 256 MB
-last-level cache) so the timed figure is HBM-bound, not cache-resident.
+Beside it: the 3-D single-layer BEM matrix (n=32768 jittered Fibonacci
+sphere, support boxes of half-width h, X 32768 x 64), a batch of resident
+stores of uneven size (one launch chain, and one HIP graph) and a served
+load with continuous batching.  The CPU leg times the reference's own
+compiled matmul (oracle/_ref) on one core (the headline baseline) and on all
+host cores one column per thread; its outputs are the FP64 references every
+GPU result's relative error is taken against.
 
 Prints one JSON line: value = algorithmic bytes per apply / time per apply
-(GB/s) for the FP32 split-precision apply at k=1; the FP64 apply and k=32 are
-reported beside it, with relative errors against the FP64 CPU reference.
+(GB/s) for the FP32 split-precision apply at k=1 of the headline workload.
 """
 
 from __future__ import annotations
@@ -33,7 +40,6 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 T0 = time.time()
-
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "apply_square_exp_16384_hbm_traffic.md"
 
@@ -56,15 +62,26 @@ def log(msg: str) -> None:
     print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_facts() -> dict:
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "model": model}
+
+
 def algorithmic_bytes(stats: dict, k: int) -> dict:
     """Bytes an apply must move at minimum: the packed words as the reference
     stores them (per block and component, not per tile), two FP64 scales and
     a width per component, the diagonal blocks, x and y once and the
-    permutation (8 bytes per index, as the reference holds it).  Split by
-    pass: pass 1 reads the right factors (and their widths), its share of the
-    fused blocks and diagonal blocks, and x; pass 2 the remaining left factors
-    (with the scales) and its share of the fused and diagonal blocks, and
-    writes y."""
+    permutation (8 bytes per index, as the reference holds it).  Split by the
+    narrow passes: pass 1 reads the right factors (and their widths), the
+    fused blocks and the diagonal blocks, and x; pass 2 the remaining left
+    factors (with the scales) and writes y."""
     n = stats["order"]
     comps = stats["ncomponents"]
     fused = stats["fused_components"]
@@ -84,10 +101,36 @@ def flops(stats: dict, k: int) -> dict:
             "fp64_dense": stats["dense_flops_per_rhs"] * k}
 
 
+class Leg:
+    """One workload: its host store, resident device copies and results."""
+
+    def __init__(self, name: str, workload, copies: int):
+        self.name = name
+        self.workload = workload
+        t = time.time()
+        last = [0.0]
+
+        def progress(done, total):
+            if time.time() - last[0] > 10 or done == total:
+                last[0] = time.time()
+                log(f"{name}: {done}/{total} blocks factored")
+
+        self.h = workload.build(progress=progress)
+        self.build_s = time.time() - t
+        self.native = self.h.native
+        self.stores = [self.native.device_store(0)] + [self.native.upload_copy(0)
+                                                       for _ in range(copies - 1)]
+        self.copies = copies
+        self.stats = self.native.device_stats(0)
+        self.results = {}  # (k, precision) -> dict
+        log(f"{name}: built in {self.build_s:.1f}s, {self.stats['nblocks']} blocks, "
+            f"{4 * self.stats['code_words'] / 1e6:.1f} MB code stream, {copies} copies resident")
+
+
 def time_applies(torch, lib, stores, x, y, ws, k, precision, steps, warmup):
     """Runs warmup + steps applies cycling through `stores`; returns (ms per
-    apply by events around the loop, per-pass ms from library events).  The
-    per-pass events are recorded in a second loop: they sit on the stream
+    apply by events around the loop, per-pass ms from the library's events).
+    The per-pass events are recorded in a second loop: they sit on the stream
     between the launches and would lengthen the timed one."""
     import ctypes as C
 
@@ -123,35 +166,110 @@ def time_applies(torch, lib, stores, x, y, ws, k, precision, steps, warmup):
     return total_ms / steps, [p / max(count, 1) for p in passes]
 
 
-def cpu_baseline(h, x1, budget_s: float):
-    """The reference's compiled CPU matmul (oracle/_ref) when present, else the
-    oracle restatement, on one core, k=1; returns (summary, y)."""
+def run_leg(torch, leg: Leg, ks, steps: int, warmup: int) -> None:
+    from qhmat_amd import _native as N
+    from qhmat_amd import device as D
+
+    lib = N.lib()
+    for k in ks:
+        x_np = leg.workload.rhs_matrix(k)
+        x = torch.from_numpy(x_np).cuda()
+        y = torch.empty_like(x)
+        ws = torch.empty(max(D.workspace_bytes(leg.native, k), 8), dtype=torch.uint8,
+                         device="cuda")
+        for prec_name, prec in (("fp32split", N.PRECISION_FP32SPLIT), ("fp64", N.PRECISION_FP64)):
+            ms, passes = time_applies(torch, lib, leg.stores, x, y, ws, k, prec, steps, warmup)
+            out = leg.native.apply_device(x, precision=prec_name, workspace=ws)
+            torch.cuda.synchronize()
+            leg.results[(k, prec_name)] = {"ms": ms, "passes_ms": passes, "y": out.cpu().numpy()}
+            log(f"{leg.name} k={k} {prec_name}: {ms * 1e3:.1f} us/apply (passes "
+                f"{passes[0] * 1e3:.1f} + {passes[1] * 1e3:.1f} us with timing events)")
+        del x, y, ws
+
+
+def cpu_leg(legs: dict, headline: Leg, cpu_seconds: float) -> dict:
+    """The reference's compiled matmul (oracle/_ref; the oracle restatement
+    when that is not built): on one core at k=1 on the headline store (the
+    baseline), and on all host cores, one right-hand-side column per thread,
+    at every right-hand-side count the GPU legs ran.  Returns the timings and
+    the outputs (the FP64 references for the GPU results' errors)."""
     from oracle import oracle as O
 
-    tmp = tempfile.mkdtemp(prefix="qhmat-bench-")
-    path = os.path.join(tmp, "store.qhmat")
-    h.save(path)
-    try:
-        if O.ref_available():
-            store, kind = O.RefHMatrix.load(path), "reference"
-        else:
-            store, kind = O.OracleStore.from_file(path, verify_checksum=False), "port"
-    finally:
-        os.remove(path)
-        os.rmdir(tmp)
-    times, y = [], None
-    t_end = time.time() + budget_s
-    while len(times) < 3 or (time.time() < t_end and len(times) < 50):
-        t = time.perf_counter()
-        y = store.matmul(x1)
-        times.append(time.perf_counter() - t)
-    return {"kind": kind, "seconds": statistics.median(times), "runs": len(times)}, y
+    facts = cpu_facts()
+    out = {"facts": facts, "all_cores": {}, "y": {}}
+    for name, leg in legs.items():
+        tmp = tempfile.mkdtemp(prefix="qhmat-bench-")
+        path = os.path.join(tmp, "store.qhmat")
+        leg.h.save(path)
+        try:
+            if O.ref_available():
+                store, kind = O.RefHMatrix.load(path), "reference"
+            else:
+                store, kind = O.OracleStore.from_file(path, verify_checksum=False), "port"
+        finally:
+            os.remove(path)
+            os.rmdir(tmp)
+        out["kind"] = kind
+        if leg is headline:
+            x1 = leg.workload.rhs_matrix(1)
+            times, y = [], None
+            t_end = time.time() + cpu_seconds
+            while len(times) < 3 or (time.time() < t_end and len(times) < 50):
+                t = time.perf_counter()
+                y = store.matmul(x1)
+                times.append(time.perf_counter() - t)
+            out["one_core"] = {"seconds": statistics.median(times), "runs": len(times)}
+            out["y"][(name, 1)] = y
+            log(f"cpu baseline ({kind}, 1 core): {out['one_core']['seconds'] * 1e3:.1f} ms/apply "
+                f"on {name} at k=1")
+        for k in sorted({k for (k, _) in leg.results}):
+            if (name, k) in out["y"]:
+                continue
+            x = leg.workload.rhs_matrix(k)
+            t = time.perf_counter()
+            out["y"][(name, k)] = store.matmul_columns(x, workers=min(k, facts["nproc"], 16))
+            sec = time.perf_counter() - t
+            out["all_cores"][f"{name}_k{k}"] = {
+                "seconds": round(sec, 3), "threads": min(k, facts["nproc"], 16),
+                "matvecs_per_s": round(k / sec, 2)}
+            log(f"cpu ({kind}, {min(k, facts['nproc'], 16)} threads): {name} k={k} in {sec:.2f}s")
+        del store
+    return out
 
 
-def batch_stores(count: int, seed: int = 4):
-    """Stores of uneven order, kernel, dimension and tolerance (a reduced
-    version of the 64-store batch: n log-uniform in [1024, 4096]) with
-    right-hand-side counts uniform in 1..16."""
+def summary(leg: Leg, k: int, prec: str, y_ref) -> dict:
+    import numpy as np
+
+    r = leg.results[(k, prec)]
+    b = algorithmic_bytes(leg.stats, k)
+    f = flops(leg.stats, k)
+    t = r["ms"] * 1e-3
+    peak = FP32_PEAK_TFLOPS if prec == "fp32split" else FP64_PEAK_TFLOPS
+    t_roof = max(b["total"] / (HBM_PEAK_GBS * 1e9),
+                 f["fp_codes"] / (peak * 1e12) + f["fp64_dense"] / (FP64_PEAK_TFLOPS * 1e12))
+    out = {
+        "ms_per_apply": r["ms"],
+        "gbps": b["total"] / t / 1e9,
+        "code_stream_gbps": b["code_stream"] / t / 1e9,
+        "matvecs_per_s": k / t,
+        "tflops": (f["fp_codes"] + f["fp64_dense"]) / t / 1e12,
+        "pass_ms": r["passes_ms"],
+        "t_roof_ms": t_roof * 1e3,
+        "roofline_frac": t_roof / t,
+    }
+    if y_ref is not None:
+        yy = r["y"].reshape(-1)
+        out["rel_err_vs_cpu_fp64"] = float(np.linalg.norm(yy - y_ref.reshape(-1))
+                                           / np.linalg.norm(y_ref))
+    return {a: (float(f"{v:.6g}") if isinstance(v, float) else v) for a, v in out.items()}
+
+
+def batch_stores(count: int, seed: int = 4, n_max: int = 16384):
+    """Stores of uneven order, kernel, dimension and tolerance: n log-uniform
+    in [1024, n_max], kernels 1/(r+0.4), exp(-r/0.2), log(r+1e-2) on 1-D, 2-D
+    and 3-D uniform points, tolerances 1e-4 and 1e-6 (1e-8 is infeasible for
+    the reference constructor at these sizes), right-hand-side counts uniform
+    in 1..16."""
     import numpy as np
 
     from qhmat_amd import workloads as W
@@ -160,7 +278,7 @@ def batch_stores(count: int, seed: int = 4):
     kernels = [W.inverse_shifted, W.exponential, W.log_kernel]
     natives, ks, shapes = [], [], []
     for i in range(count):
-        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(4096)))))
+        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(n_max)))))
         dim = 1 + i % 3
         tol = (1e-4, 1e-6)[i % 2]
         pts = W.uniform_points(n, dim, seed=100 + i)
@@ -173,8 +291,8 @@ def batch_stores(count: int, seed: int = 4):
 
 def measure_batch(torch, natives, ks, repeats: int = 50) -> dict:
     """Tunes the stores, then times the batch three ways: every store applied
-    on its own, the batched launch pair per class, and the same batch
-    replayed as one captured HIP graph (FP32 split)."""
+    on its own, the batched launch chain, and the same batch replayed as one
+    captured HIP graph (FP32 split)."""
     import ctypes as C
 
     from qhmat_amd import _native as N
@@ -253,6 +371,9 @@ def main() -> int:
                     help="first copy 256 MiB device-to-device once (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--rhs", type=int, nargs="+", default=[1, 32])
+    ap.add_argument("--sphere-n", type=int, default=32768,
+                    help="order of the 3-D BEM workload (0 skips it)")
+    ap.add_argument("--sphere-rhs", type=int, nargs="+", default=[1, 64])
     ap.add_argument("--batch-stores", type=int, default=12,
                     help="stores in the batched-graph measurement (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
@@ -273,8 +394,6 @@ def main() -> int:
     import numpy as np
     import torch
 
-    import qhmat_amd as q
-    from qhmat_amd import _native as N
     from qhmat_amd import workloads as W
 
     if not torch.cuda.is_available():
@@ -288,30 +407,15 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
         log("calibration copy: 268435456 bytes read, 268435456 bytes written")
-    log(f"building square_exp n={args.n}")
-    w = W.square_exp(args.n)
-    h = w.build()
-    native = h.native
-    stores = [native.device_store(0)] + [native.upload_copy(0) for _ in range(args.copies - 1)]
-    stats = native.device_stats(0)
-    log(f"store: {stats['nblocks']} blocks, {stats['codes']} codes, "
-        f"{4 * stats['code_words'] / 1e6:.1f} MB code stream, {args.copies} copies resident")
-    lib = N.lib()
-    results = {}
-    for k in args.rhs:
-        x_np = w.rhs_matrix(k)
-        x = torch.from_numpy(x_np).cuda()
-        y = torch.empty_like(x)
-        from qhmat_amd import device as D
 
-        ws = torch.empty(max(D.workspace_bytes(native, k), 8), dtype=torch.uint8, device="cuda")
-        for prec_name, prec in (("fp32split", N.PRECISION_FP32SPLIT), ("fp64", N.PRECISION_FP64)):
-            ms, passes = time_applies(torch, lib, stores, x, y, ws, k, prec, args.steps,
-                                      args.warmup)
-            out = native.matmul(x_np, True, precision=prec_name)
-            results[(k, prec_name)] = {"ms": ms, "passes_ms": passes, "y": out}
-            log(f"k={k} {prec_name}: {ms * 1e3:.1f} us/apply "
-                f"(passes {passes[0] * 1e3:.1f} + {passes[1] * 1e3:.1f} us with timing events)")
+    legs = {}
+    head = Leg("square_exp", W.square_exp(args.n), args.copies)
+    legs[head.name] = head
+    run_leg(torch, head, args.rhs, args.steps, args.warmup)
+    if args.sphere_n > 0:
+        sphere = Leg("sphere_bem", W.sphere_bem(args.sphere_n), 1)
+        legs[sphere.name] = sphere
+        run_leg(torch, sphere, args.sphere_rhs, max(args.steps // 4, 10), max(args.warmup // 4, 3))
 
     batched = None
     if args.batch_stores > 0:
@@ -323,7 +427,7 @@ def main() -> int:
     serving = None
     if args.serve_seconds > 0:
         log("serving: building 3 smaller stores")
-        served = {"square_exp_16384": native}
+        served = {"square_exp_16384": head.native}
         for m in (2048, 4096, 8192):
             served[f"square_exp_{m}"] = W.square_exp(m).build().native
         serving = []
@@ -338,45 +442,22 @@ def main() -> int:
         from qhmat_amd.harness import Harness
 
         log("harness: reconstructing and decoding the store")
-        hn = Harness(native)
+        hn = Harness(head.native)
         harness = {f"k{k}": hn.run(k) for k in (1, 32)}
         for kk, res in harness.items():
             log(f"harness {kk}: " + ", ".join(f"{m} {v['ms'] * 1e3:.1f} us" for m, v in res.items()))
         del hn
 
-    base = None
-    y_cpu = None
+    cpu = None
     if not args.no_cpu:
-        log("timing the CPU baseline (1 core, k=1)")
-        base, y_cpu = cpu_baseline(h, w.rhs_matrix(1), args.cpu_seconds)
-        log(f"cpu baseline ({base['kind']}): {base['seconds'] * 1e3:.1f} ms/apply")
+        cpu = cpu_leg(legs, head, args.cpu_seconds)
 
-    def summary(k, prec):
-        r = results[(k, prec)]
-        b = algorithmic_bytes(stats, k)
-        f = flops(stats, k)
-        t = r["ms"] * 1e-3
-        out = {
-            "ms_per_apply": r["ms"],
-            "gbps": b["total"] / t / 1e9,
-            "code_stream_gbps": b["code_stream"] / t / 1e9,
-            "matvecs_per_s": k / t,
-            "pass_ms": r["passes_ms"],
-            "t_roof_ms": max(b["total"] / (HBM_PEAK_GBS * 1e9),
-                             f["fp_codes"] / ((FP32_PEAK_TFLOPS if prec == "fp32split"
-                                               else FP64_PEAK_TFLOPS) * 1e12)
-                             + f["fp64_dense"] / (FP64_PEAK_TFLOPS * 1e12)) * 1e3,
-        }
-        out["roofline_frac"] = out["t_roof_ms"] / r["ms"]
-        if k == 1 and y_cpu is not None:
-            yy = r["y"].reshape(-1)
-            out["rel_err_vs_cpu_fp64"] = float(np.linalg.norm(yy - y_cpu.reshape(-1))
-                                               / np.linalg.norm(y_cpu))
-        return out
+    def y_ref(leg, k):
+        return None if cpu is None else cpu["y"].get((leg.name, k))
 
-    head = summary(1, "fp32split")
-    b1 = algorithmic_bytes(stats, 1)
-    r1 = results[(1, "fp32split")]
+    b1 = algorithmic_bytes(head.stats, 1)
+    r1 = head.results[(1, "fp32split")]
+    hs = summary(head, 1, "fp32split", y_ref(head, 1))
     dominant = int(np.argmax(r1["passes_ms"]))
     dom_ms = r1["passes_ms"][dominant]
     dom_bytes = b1[f"pass{dominant}"]
@@ -384,7 +465,7 @@ def main() -> int:
     dom_kernel = ["apply_pass1<1, true>", "apply_pass2<1, true>"][dominant]
     line = {
         "metric": "H-matvec GB/s (% HBM peak) FP32 vs FP64 at rel err; served p50/p99 ms",
-        "value": round(head["gbps"], 2),
+        "value": round(hs["gbps"], 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -395,21 +476,31 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32 (codes x vector, split for 32-bit codes), fp64 scales/diagonal/accumulation",
         "data": "synthetic (seeded points and N(0,1) right-hand sides)",
-        "config": {"workload": "square_exp: n=16384 uniform 2-D, exp(-r/0.2), leaf 64, "
+        "config": {"workload": f"square_exp: n={args.n} uniform 2-D, exp(-r/0.2), leaf 64, "
                                "tol 1e-6, widths 4..32, k=1, 4 rotated store copies",
                    "n": args.n, "rhs": 1, "copies": args.copies,
                    "store_code_stream_bytes": b1["code_stream"],
                    "algorithmic_bytes_per_apply": b1["total"]},
-        "pct_hbm_peak": round(100 * head["gbps"] / HBM_PEAK_GBS, 2),
+        "pct_hbm_peak": round(100 * hs["gbps"] / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom_kernel),
                      "kernel": dom_kernel,
                      "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
     }
-    for k in args.rhs:
-        for prec in ("fp32split", "fp64"):
-            line[f"{prec}_k{k}"] = summary(k, prec)
+    for leg in legs.values():
+        entries = {}
+        for (k, prec) in sorted(leg.results):
+            entries[f"{prec}_k{k}"] = summary(leg, k, prec, y_ref(leg, k))
+        for k in sorted({k for (k, _) in leg.results}):
+            a, b = entries[f"fp32split_k{k}"], entries[f"fp64_k{k}"]
+            entries[f"fp32split_over_fp64_k{k}"] = round(a["matvecs_per_s"] / b["matvecs_per_s"], 3)
+        if leg is head:
+            line.update(entries)
+        else:
+            line[leg.name] = {"n": leg.stats["order"], "build_s": round(leg.build_s, 1),
+                              "blocks": leg.stats["nblocks"],
+                              "code_stream_bytes": 4 * leg.stats["code_words"], **entries}
     if batched is not None:
         line["batched_graph"] = batched
     if serving:
@@ -420,17 +511,15 @@ def main() -> int:
         line["served_rate_rps"] = serving[-1]["offered_rate"]
     if harness is not None:
         line["harness"] = harness
-    if base is not None:
+    if cpu is not None:
+        one = cpu["one_core"]
         line["cpu_baseline"] = {
-            "value": round(b1["total"] / base["seconds"] / 1e9, 4), "unit": "GB/s",
-            "cores": 1, "kind": base["kind"],
-            "sample": f"{base['runs']} applies of the same store at k=1 (median "
-                      f"{base['seconds'] * 1e3:.1f} ms/apply)"}
-    for k in args.rhs:
-        for prec in ("fp32split", "fp64"):
-            key = f"{prec}_k{k}"
-            line[key] = {a: (float(f"{v:.6g}") if isinstance(v, float) else v)
-                         for a, v in line[key].items()}
+            "value": round(b1["total"] / one["seconds"] / 1e9, 4), "unit": "GB/s",
+            "cores": 1, "kind": cpu["kind"],
+            "sample": f"{one['runs']} applies of the headline store at k=1 (median "
+                      f"{one['seconds'] * 1e3:.1f} ms/apply)",
+            "host_cpu": cpu["facts"]["model"], "host_nproc": cpu["facts"]["nproc"]}
+        line["cpu_all_cores"] = cpu["all_cores"]
     print(json.dumps(line), flush=True)
     return 0
 

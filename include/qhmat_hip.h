@@ -112,14 +112,17 @@ qh_status qh_device_count(int *count);
 /* Uploads a finalized store to `device` once.  Replaces the per-call flatten +
  * upload at cuda.cu:119-150. */
 qh_status qh_store_create(const qh_store_desc *desc, int device, qh_store **out);
+/* Refused (QH_EINVAL, the store kept) while a batch built on it exists. */
 qh_status qh_store_destroy(qh_store *store);
 qh_status qh_store_get_stats(const qh_store *store, qh_store_stats *out);
 
 /* Launch-shape choice (no counterpart in the reference): applies of `k`
- * right-hand sides run in passes of `kb` in {1, 2, 4, 8, 16} (0 restores the
- * default, k rounded up to a power of two, at most 16).  Results are bitwise
- * the same for every kb; the autotuner (qhmat_amd/tune.py) times the choices
- * and keeps the fastest.  Batches created afterwards group members by it. */
+ * right-hand sides run in passes of `kb` right-hand sides, kb in {1, 2, 4}
+ * (vector ALUs) or {16, 32, 64} (matrix cores); 0 restores the default (k
+ * rounded up to one of them, 64 for k > 32).  Results are bitwise the same for
+ * every kb of one of the two groups and agree to the precision's tolerance
+ * across them; the autotuner (qhmat_amd/tune.py) times the choices and keeps
+ * the fastest.  Batches created afterwards group members by it. */
 qh_status qh_store_set_rhs_block(qh_store *store, int64_t k, int kb);
 
 /* Workspace bytes qh_apply needs for `k` right-hand sides. */
@@ -162,6 +165,8 @@ qh_status qh_noalloc_scratch_bytes(uint64_t *bytes);
 typedef struct qh_batch qh_batch;
 qh_status qh_batch_create(const qh_store *const *stores, const int64_t *k, uint64_t count,
                           qh_batch **out);
+/* Refused (QH_EINVAL, the batch kept) while a graph captured from it exists;
+ * a batch keeps its member stores from being destroyed until it is. */
 qh_status qh_batch_destroy(qh_batch *batch);
 /* offsets: count entries (doubles); total: doubles in x (and y). */
 qh_status qh_batch_layout(const qh_batch *batch, uint64_t *offsets, uint64_t *total,
@@ -197,6 +202,11 @@ qh_status qh_profile_end(qh_store *store, qh_profile *out);
  * access bounds-checked, writing y = H x (FP64, order x k row-major).  A
  * test hook for the tile-major layout; no apply path calls it. */
 qh_status qh_plan_emulate(const qh_store_desc *desc, const double *x, int64_t k, double *y);
+/* The same check for the tables of the wide (16/32/64 right-hand sides per
+ * pass, matrix-core) passes, whose partial-sum rows follow the precision's
+ * order of virtual rows (`precision` a qh_precision). */
+qh_status qh_plan_emulate_wide(const qh_store_desc *desc, const double *x, int64_t k, double *y,
+                               int precision);
 
 /* Dense reconstruction on the device: writes the order x order FP64 matrix
  * in source index order into device memory `out` (row-major, leading

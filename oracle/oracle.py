@@ -188,6 +188,23 @@ class OracleStore:
             raise ValueError("nonfinite right-hand side")
         return y.reshape(self.order) if vec else y
 
+    def matmul_columns(self, x: np.ndarray, workers: int | None = None) -> np.ndarray:
+        """matmul one right-hand-side column at a time on a thread pool.
+
+        In the reference loop nest (core.cpp:360-390) column p's arithmetic
+        reads and writes column p only, in an order that does not depend on
+        the column count, so the result is bitwise matmul(x); the columns run
+        in parallel because ctypes releases the interpreter lock."""
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+
+        a = np.asarray(x, dtype=np.float64).reshape(self.order, -1)
+        cols = [np.ascontiguousarray(a[:, p]) for p in range(a.shape[1])]
+        workers = workers or min(len(cols), os.cpu_count() or 1, 16)
+        with ThreadPoolExecutor(max_workers=max(workers, 1)) as pool:
+            out = list(pool.map(self.matmul, cols))
+        return np.stack(out, axis=1).reshape(np.shape(x))
+
     def reconstruct(self) -> np.ndarray:
         out = np.zeros((self.order, self.order))
         lib().orc_reconstruct(C.byref(self._c), _ptr(out))
@@ -294,6 +311,20 @@ class RefHMatrix:
         y = np.zeros_like(a2)
         _ref_check(ref().ref_matmul(self._h, _ptr(a2), a2.shape[1], _ptr(y)))
         return y.reshape(a.shape)
+
+    def matmul_columns(self, x: np.ndarray, workers: int | None = None) -> np.ndarray:
+        """matmul one column at a time on a thread pool: bitwise matmul(x),
+        since core.cpp:360-390 reads and writes only column p for column p
+        (the same argument as OracleStore.matmul_columns)."""
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+
+        a = np.asarray(x, dtype=np.float64).reshape(self.order, -1)
+        cols = [np.ascontiguousarray(a[:, p]) for p in range(a.shape[1])]
+        workers = workers or min(len(cols), os.cpu_count() or 1, 16)
+        with ThreadPoolExecutor(max_workers=max(workers, 1)) as pool:
+            out = list(pool.map(self.matmul, cols))
+        return np.stack(out, axis=1).reshape(np.shape(x))
 
     def reconstruct(self) -> np.ndarray:
         out = np.zeros((self.order, self.order))

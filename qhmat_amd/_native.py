@@ -113,6 +113,7 @@ _sig("qh_noalloc_scratch_bytes", C.POINTER(_u64))
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
 _sig("qh_reconstruct", _p, _p, _i64, _p)
 _sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)
+_sig("qh_plan_emulate_wide", C.POINTER(StoreDesc), _p, _i64, _p, C.c_int)
 _sig("qh_store_set_rhs_block", _p, _i64, C.c_int)
 _sig("qh_profile_begin", _p, _u64)
 _sig("qh_profile_end", _p, C.POINTER(Profile))
@@ -305,7 +306,7 @@ class NativeHMatrix:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _lib is not None:
             for store in getattr(self, "_dev", {}).values():
                 _lib.qh_store_destroy(store)
             self._dev = {}
@@ -480,14 +481,19 @@ class NativeHMatrix:
         check(_lib.qh_store_create(C.byref(desc), int(device), C.byref(handle)))
         return handle
 
-    def plan_emulate(self, x) -> np.ndarray:
-        """Host check of the device layout (qh_plan_emulate): y = H x computed
-        by replaying the kernels' index arithmetic with bounds checks."""
+    def plan_emulate(self, x, wide: bool = False, precision: str = "fp64") -> np.ndarray:
+        """Host check of the device layout (qh_plan_emulate, or for the wide
+        passes qh_plan_emulate_wide in `precision`'s virtual-row order): y = H x
+        computed by replaying the kernels' index arithmetic with bounds checks."""
         a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(self.order, -1))
-        y = np.zeros_like(a)
+        y = np.full_like(a, np.nan)
         desc = StoreDesc()
         check(_lib.qh_host_describe(self._h, C.byref(desc)))
-        check(_lib.qh_plan_emulate(C.byref(desc), _ptr(a), a.shape[1], _ptr(y)))
+        if wide:
+            prec = PRECISION_FP32SPLIT if precision == "fp32split" else PRECISION_FP64
+            check(_lib.qh_plan_emulate_wide(C.byref(desc), _ptr(a), a.shape[1], _ptr(y), prec))
+        else:
+            check(_lib.qh_plan_emulate(C.byref(desc), _ptr(a), a.shape[1], _ptr(y)))
         return y
 
     def device_stats(self, device: int = 0) -> dict:
@@ -496,11 +502,13 @@ class NativeHMatrix:
         return stats.as_dict()
 
     def release_device(self, device: int | None = None) -> None:
-        """Frees the resident copy on one device (or all)."""
+        """Frees the resident copy on one device (or all).  Raises ValueError,
+        keeping the copy, while a batch built on it exists."""
         with self._dev_lock:
             for d in list(self._dev):
                 if device is None or d == device:
-                    _lib.qh_store_destroy(self._dev.pop(d))
+                    check(_lib.qh_store_destroy(self._dev[d]))
+                    del self._dev[d]
 
     def _matmul_device(self, a: np.ndarray, rhs: int, precision: str) -> np.ndarray:
         from . import device as _device

@@ -45,7 +45,7 @@ class Graph:
         N.check(_lib.qh_graph_launch(self._h, _p(_stream_handle(stream))))
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.qh_graph_destroy(self._h)
             self._h = None
 
@@ -73,7 +73,7 @@ class DeviceBatch:
         self.orders = [n.order for n in natives]
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.qh_batch_destroy(self._h)
             self._h = None
 

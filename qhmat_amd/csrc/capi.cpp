@@ -38,6 +38,7 @@ struct qh_batch {
 
 struct qh_graph {
     void* exec = nullptr;  // hipGraphExec_t
+    qhamd::Batch* batch = nullptr;  // pinned while the graph exists
 };
 
 namespace {
@@ -214,7 +215,8 @@ qh_status qh_batch_graph_create(const qh_batch* batch, const double* x, double* 
         *out = nullptr;
         void* exec = qhamd::batch_graph(*batch->batch, x, y, precision, workspace,
                                         workspace_bytes);
-        *out = new qh_graph{exec};
+        batch->batch->pins.fetch_add(1);
+        *out = new qh_graph{exec, batch->batch};
     });
 }
 
@@ -229,6 +231,7 @@ qh_status qh_graph_destroy(qh_graph* graph) {
     return guarded([&] {
         if (!graph) return;
         qhamd::graph_destroy(graph->exec);
+        if (graph->batch) graph->batch->pins.fetch_sub(1);
         delete graph;
     });
 }
@@ -241,6 +244,20 @@ qh_status qh_plan_emulate(const qh_store_desc* desc, const double* x, int64_t k,
         if (k <= 0) throw std::invalid_argument("right-hand-side shape mismatch");
         const qhamd::Plan plan = qhamd::build_plan(*desc);
         qhamd::emulate_plan(plan, x, k, y);
+    });
+}
+
+qh_status qh_plan_emulate_wide(const qh_store_desc* desc, const double* x, int64_t k,
+                               double* y, int precision) {
+    return guarded([&] {
+        need(desc, "description");
+        need(x, "right-hand side");
+        need(y, "result");
+        if (k <= 0) throw std::invalid_argument("right-hand-side shape mismatch");
+        const qhamd::Plan plan = qhamd::build_plan(*desc);
+        if (precision != QH_PRECISION_FP64 && precision != QH_PRECISION_FP32SPLIT)
+            throw std::invalid_argument("unknown precision mode");
+        qhamd::emulate_wide(plan, x, k, y, precision == QH_PRECISION_FP32SPLIT);
     });
 }
 

@@ -40,110 +40,33 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <set>
+#include <tuple>
+#include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 
+#include "device_common.hpp"
 #include "device_store.hpp"
+#include "wide_passes.hpp"
 
 namespace qhamd {
 namespace {
 
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Makes `device` current for the scope and restores the caller's device.
-struct DeviceScope {
-    int previous = -1;
-    explicit DeviceScope(int device) {
-        hip_check(hipGetDevice(&previous), "hipGetDevice");
-        if (previous != device) hip_check(hipSetDevice(device), "hipSetDevice");
-    }
-    ~DeviceScope() {
-        int now = -1;
-        if (hipGetDevice(&now) == hipSuccess && now != previous && previous >= 0)
-            (void)hipSetDevice(previous);
-    }
-};
-
-void require(bool ok, const char* message) {
-    if (!ok) throw std::invalid_argument(message);
-}
-
-// ---------------------------------------------------------------------------
-// Device helpers
-
-// Slot of lane L in word row w of a side block.  Right-factor blocks are
-// read by lanes that sit on different rows at the same columns (a lane per
-// component), so their rows are swizzled: lane L at L xor 4 (w mod 16),
-// which keeps every aligned group of four columns together and puts the
-// same columns of different rows on different banks.  Left-factor blocks
-// are read a whole row at a time and keep L.
-__host__ __device__ __forceinline__ std::uint32_t lane_slot(bool swz, std::uint64_t w,
-                                                            std::uint64_t L) {
-    return std::uint32_t(swz ? (L ^ (4u * (w & 15u))) : L);
-}
-
-// Code of lane `lane` at bit offset `o` of a side block at `side` (global
-// memory).  The next row of the block is read only when the code straddles
-// into it.
-__device__ __forceinline__ int decode_side_code(const std::uint32_t* side, std::uint32_t lane,
-                                                std::uint32_t o, std::uint32_t bits, bool swz) {
-    const std::uint32_t k = o >> 5, sh = o & 31u;
-    const std::uint32_t lo = side[64u * k + lane_slot(swz, k, lane)];
-    const std::uint32_t hi =
-        sh + bits > 32u ? side[64u * (k + 1u) + lane_slot(swz, k + 1u, lane)] : 0u;
-    const std::uint32_t raw = __builtin_amdgcn_alignbit(hi, lo, sh);
-    return bits == 32 ? static_cast<int>(raw) : __builtin_amdgcn_sbfe(raw, 0u, bits);
-}
-
-
-// LDS written by some lanes of a wave and read by others: keep the compiler
-// from moving accesses across this point and wait for the LDS queue.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Inclusive prefix sum over the 64 lanes, all in DPP: shifted adds within
-// each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and
-// row 2's into row 3 (row_bcast15), then rows 0-1's total into rows 2-3
-// (row_bcast31).  A lane without a source adds the zero passed as `old`.
-__device__ __forceinline__ std::uint32_t wave_scan(std::uint32_t v, int) {
-    int x = int(v);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
-    return std::uint32_t(x);
-}
-
-// 16-byte group as a native clang vector (an array of HIP's union-based
-// uint4 would be kept in private scratch memory by the compiler).
-typedef std::uint32_t quad_t __attribute__((ext_vector_type(4)));
-
-template <bool FP32> using real_t = typename std::conditional<FP32, float, double>::type;
 
 // LDS of one wave: a head word that decoding may read below the first
 // stream, the current item's data plus slack that lanes decoding past the end
 // of a stream may read (64 codes of 32 bits and the word after), then the
 // item's inputs (x rows of its columns, or the workspace partial sums of
-// kLeft), KB values per lane, which also take the item's rows at its end,
-// then the scaled inputs of 16 components (zt), then the item's scale
-// products and widths (lanes = components).
-constexpr int kStageBytes = kItemQuads * 16 + 272;
-constexpr int kLdsHead = 256;  // readable bytes before the stage (decode_lm)
-template <int KB> constexpr int wave_lds_meta() {
-    return kLdsHead + kStageBytes + kTile * KB * 8 + 16 * KB * 8;
-}
+// kLeft), KB values per lane, then the item's scale products and widths
+// (lanes = components).
+template <int KB> constexpr int wave_lds_meta() { return kLdsHead + kStageBytes + kTile * KB * 8; }
 // Narrow passes in FP32 mode also keep the x rows in FP32 (lanes = columns).
 template <int KB> constexpr int wave_lds_xf() { return wave_lds_meta<KB>() + 64 * 12; }
 template <int KB> constexpr int wave_lds_bytes() {
-    return wave_lds_xf<KB>() + (KB < 8 ? 64 * KB * 4 : 0);
+    return wave_lds_xf<KB>() + 64 * KB * 4;
 }
 
 // Uniform fields of item `slot` of the record group held in r0/r1 (lanes =
@@ -188,22 +111,6 @@ template <int KB> struct Fetch {
     double in[KB];       // x rows (lanes = columns) or partial sums (lanes = pairs)
 };
 
-// Loads go through buffer resources (V#) sized to the bytes a request may
-// touch: lanes outside an item (or past its columns / components) use an
-// out-of-range offset and read zero, so no load is predicated and every
-// load of an item is issued, whatever its shape (the compiler then counts
-// the loads in flight exactly, which lets two items be in flight).
-constexpr int kBufferFlags = 0x00020000;      // dword 3 of a raw 32-bit buffer resource
-constexpr std::uint32_t kOutOfRange = 0x7FFFFFF0u;  // >= every num_records used here
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void* base, std::uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), kBufferFlags);
-}
-__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, std::uint32_t off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    return __longlong_as_double(static_cast<long long>(
-        (static_cast<unsigned long long>(v[1]) << 32) | static_cast<unsigned long long>(v[0])));
-}
 
 // Row sets per tile in the workspace (four workgroups per pass), and x in
 // clustered order after them: xc[i][p] = x[perm[i]][p], written once per
@@ -265,81 +172,8 @@ __device__ __forceinline__ std::uint32_t side_words(std::uint32_t total) {
     return 64u * ((total + 31u) >> 5);
 }
 
-// Matrix cores.  v_mfma_f32_16x16x4f32 and v_mfma_f64_16x16x4f64 form
-// D += A B with A 16 x 4 and B 4 x 16: lane l = 16 q + r supplies A[r][q] and
-// B[q][r], and slot t of its result holds D[4q + t][r] (FP32) or D[4t + q][r]
-// (FP64).  In FP64 mode lane r supplies row a_row(r) of A instead, which puts
-// D[4q + t][r] in slot t as well, so both modes share one result layout.
-// Each result element is its own dot product: column r of D depends on
-// column r of B only, so a right-hand side's arithmetic does not depend on
-// the others in its pass.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-// Right-hand sides per pass from which the matrix cores take the codes (below,
-// one code's products are too few to fill them and the vector path is faster).
-constexpr int kMatrixMinKB = 8;
-
-template <bool FP32> struct Mma;
-template <> struct Mma<true> {
-    using acc = f32x4;
-    static __device__ __forceinline__ acc zero() { return acc{0.0f, 0.0f, 0.0f, 0.0f}; }
-    static __device__ __forceinline__ acc mad(float a, float b, acc d) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
-    }
-};
-template <> struct Mma<false> {
-    using acc = f64x4;
-    static __device__ __forceinline__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
-    static __device__ __forceinline__ acc mad(double a, double b, acc d) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
-    }
-};
-template <bool FP32> __host__ __device__ __forceinline__ int a_row(int r) {
-    return FP32 ? r : 4 * (r & 3) + (r >> 2);
-}
-
-// The code of lane `lane` for the component with packed fields m in a staged
-// side block (word w of lane L at side[64 w + L]; one row of 64 readable
-// words is kept before the stage and after the item): the 32 bits [E - 32,
-// E) of the lane's stream from words (E >> 5) - 1 and E >> 5
-// (v_alignbit_b32), shifted arithmetically right by 32 - b, which
-// sign-extends the field (b = 32 included) and drops the bits below it.  No
-// mask and no branch; every lane reads its own column of words, so the 64
-// lanes of a read hit 64 consecutive words.
-__device__ __forceinline__ int decode_lm(const std::uint32_t* side, std::uint32_t m, int lane) {
-    const std::uint32_t E = m >> 6;
-    const std::uint32_t* w = side + 64u * (E >> 5) + std::uint32_t(lane);
-    return static_cast<int>(__builtin_amdgcn_alignbit(w[0], w[-64], E)) >> (32u - (m & 63u));
-}
-
-// Codes of columns c4 .. c4 + 3 (c4 a multiple of 4) for the component with
-// packed fields m in a staged right side block, whose rows are swizzled
-// (lane_slot): the four columns stay consecutive within each of the two
-// rows read, so each row is one aligned read of four words per lane.
-__device__ __forceinline__ void decode_right4(const std::uint32_t* side, std::uint32_t m, int c4,
-                                              int (&code)[4]) {
-    const int E = int(m >> 6), kk = E >> 5;
-    const std::uint32_t sh = 32u - (m & 63u);
-    const std::uint32_t* hi = side + 64 * kk + (c4 ^ (4 * (kk & 15)));
-    const std::uint32_t* lo = side + 64 * (kk - 1) + (c4 ^ (4 * ((kk - 1) & 15)));
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-        code[u] = static_cast<int>(__builtin_amdgcn_alignbit(hi[u], lo[u], std::uint32_t(E))) >> sh;
-}
-
-// A code as h 2^16 + l with l its sign-extended low 16 bits: both halves are
-// exact in FP32, and a code of small magnitude stays in l (no cancellation
-// between the halves).
-__device__ __forceinline__ void split_code(int code, float& h, float& l) {
-    // (code - l) / 2^16 without leaving 32 bits: the arithmetic high half,
-    // plus one when the low half is negative
-    h = float((code >> 16) + ((code >> 15) & 1));
-    l = float(static_cast<int>(__builtin_amdgcn_sbfe(code, 0u, 16u)));
-}
-
 // ---------------------------------------------------------------------------
-// Narrow passes (KB < kMatrixMinKB): vector ALUs.  Lanes are the item's rows
+// Narrow passes (1, 2 or 4 right-hand sides per pass): vector ALUs.  Lanes are the item's rows
 // (left factors, diagonal columns) or columns (right factors, summed over the
 // lanes by a butterfly of lane swaps).
 
@@ -584,194 +418,6 @@ __device__ __forceinline__ void valu_right_then_left(const MemberView& M, const 
 }
 
 
-// ---------------------------------------------------------------------------
-// Wide passes (KB >= kMatrixMinKB): matrix cores.
-
-// Per-item rows on the matrix cores: slot t of d[rb] in lane 16q + r holds
-// row 16 rb + 4q + t of the item's tile for right-hand side r mod KB.
-template <bool FP32> struct ItemRows {
-    typename Mma<FP32>::acc d[4];
-    __device__ __forceinline__ ItemRows() {
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) d[rb] = Mma<FP32>::zero();
-    }
-};
-
-// Left-factor rows of components [g0, g0 + 16) of an item (g0 a multiple of
-// 16) on the matrix cores, one MFMA per 16 rows x 4 components: lane 16q + r
-// decodes row 16 rb + a_row(r) of component g0 + 4s + q and supplies that
-// component's scaled input for right-hand side r mod KB from the LDS table
-// zt (zt[(c - g0) KB + p], zero past the item's components, whose dummy
-// streams decode in bounds).  Rows past the tile's decode neighbouring bits,
-// and only their own results, which are never used.  WIDE (FP32 mode, an
-// item with a code wider than 24 bits): every code is split (split_code), h
-// against 2^16 zs and l against zs, into the same sums.
-template <int KB, bool FP32, bool WIDE>
-__device__ __forceinline__ void left_group(const std::uint32_t* stage, const real_t<FP32>* zt,
-                                           std::uint32_t lmeta_l, std::uint32_t g0, int lane,
-                                           ItemRows<FP32>& rows) {
-    using Mm = Mma<FP32>;
-    using Real = real_t<FP32>;
-    const int q = lane >> 4, r = lane & 15;
-    const int row = a_row<FP32>(r);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const std::uint32_t c = g0 + 4u * std::uint32_t(s) + std::uint32_t(q);  // <= 63
-        const std::uint32_t meta = std::uint32_t(__shfl(int(lmeta_l), int(c), 64));
-        const Real zs = zt[(4 * s + q) * KB + (r & (KB - 1))];
-        if constexpr (!WIDE) {
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
-                rows.d[rb] = Mm::mad(Real(decode_lm(stage, meta, 16 * rb + row)), zs, rows.d[rb]);
-        } else {
-            const float zh = float(zs) * 65536.0f;
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                float h, l;
-                split_code(decode_lm(stage, meta, 16 * rb + row), h, l);
-                rows.d[rb] = Mma<true>::mad(h, zh, rows.d[rb]);
-                rows.d[rb] = Mma<true>::mad(l, float(zs), rows.d[rb]);
-            }
-        }
-    }
-}
-
-// Diagonal-block columns on the FP64 matrix cores: lane 16q + r supplies
-// value (row 16 rb + a_row(r), column 4s + q) of the staged column-major
-// R x cnt block (zero for columns past cnt, whose x is zero too) and x of
-// that column for right-hand side r mod KB.
-template <int KB>
-__device__ __forceinline__ void dense_rows(const double* dv, const double* xs, std::uint32_t R,
-                                           std::uint32_t cnt, int lane, ItemRows<false>& rows) {
-    const int q = lane >> 4, r = lane & 15;
-    const std::uint32_t row = std::uint32_t(a_row<false>(r));
-#pragma unroll
-    for (int s = 0; s < kDenseColumns / 4; ++s) {
-        const std::uint32_t c = 4u * std::uint32_t(s) + std::uint32_t(q);
-        const double xv = xs[c * KB + (r & (KB - 1))];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const double a = c < cnt ? dv[c * R + 16u * std::uint32_t(rb) + row] : 0.0;
-            rows.d[rb] = Mma<false>::mad(a, xv, rows.d[rb]);
-        }
-    }
-}
-
-// Adds an item's rows to the wave's row sums (lanes = rows) through the LDS
-// tile T[row][KB] (the item's input area, no longer read by then), in the
-// precision the rows were summed in.
-template <int KB, bool FP32>
-__device__ __forceinline__ void add_item_rows(const ItemRows<FP32>& rows, void* tile, int lane,
-                                              double (&acc)[KB]) {
-    using Real = real_t<FP32>;
-    Real* T = static_cast<Real*>(tile);
-    const int q = lane >> 4, r = lane & 15;
-    if (r < KB) {
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) T[(16 * rb + 4 * q + t) * KB + r] = rows.d[rb][t];
-    }
-    wave_lds_sync();
-#pragma unroll
-    for (int p = 0; p < KB; ++p) acc[p] += double(T[lane * KB + p]);
-}
-
-// Right-factor sums of components [g0, g0 + 16) of a kRight or kFused item
-// on the matrix cores, one MFMA per 4 x 16 codes: z[t] in lane 16q + r is
-//   sum over the item's 64 columns c of code(c, g0 + 4q + t) x[c][r mod KB]
-// in FP64.  MFMA m (0..15) puts columns 16q + m in its k slots: each lane
-// decodes its own component's code of column 16q + m (stepping the end bit
-// by the width) and reads that column's x from LDS.  Columns past the item's
-// hold zero x; components past its count decode the dummy stream and their
-// sums are never used.  WIDE (FP32 mode, an item with a code wider than 24
-// bits): every code is split into exact halves (split_code) summed apart and
-// joined in FP64.  Both depend only on the item, so a component's rounding
-// is the same for every right-hand-side count.
-template <int KB, bool FP32, bool WIDE>
-__device__ __forceinline__ void right_group(const std::uint32_t* stage, const double* xs,
-                                            std::uint32_t rmeta_l, std::uint32_t g0, int lane,
-                                            double (&z)[4]) {
-    using Mm = Mma<FP32>;
-    using Real = real_t<FP32>;
-    const int q = lane >> 4, r = lane & 15;
-    const std::uint32_t j = g0 + std::uint32_t(a_row<FP32>(r));  // <= 63
-    const std::uint32_t meta = std::uint32_t(__shfl(int(rmeta_l), int(j), 64));
-    const double* xcol = xs + 16 * q * KB + (r & (KB - 1));
-    if constexpr (!WIDE) {
-        typename Mm::acc d = Mm::zero();
-#pragma unroll 2
-        for (int t = 0; t < 4; ++t) {
-            int code[4];
-            decode_right4(stage, meta, 16 * q + 4 * t, code);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) d = Mm::mad(Real(code[u]), Real(xcol[(4 * t + u) * KB]), d);
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) z[t] = double(d[t]);
-    } else {
-        f32x4 dh = Mma<true>::zero(), dl = Mma<true>::zero();
-#pragma unroll 2
-        for (int t = 0; t < 4; ++t) {
-            int code[4];
-            decode_right4(stage, meta, 16 * q + 4 * t, code);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float h, l;
-                split_code(code[u], h, l);
-                const float xv = float(xcol[(4 * t + u) * KB]);
-                dh = Mma<true>::mad(h, xv, dh);
-                dl = Mma<true>::mad(l, xv, dl);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) z[t] = double(dh[t]) * 65536.0 + double(dl[t]);
-    }
-}
-
-// kFused components [g0, g0 + 16): right-factor sums, their scaled inputs
-// through the LDS table zt, and the left streams of the same components.
-template <int KB, bool FP32, bool WIDE>
-__device__ __forceinline__ void fused_group(const Item& f, const std::uint32_t* stage,
-                                            const std::uint32_t* lstage, const double* xs,
-                                            real_t<FP32>* zt, std::uint32_t rmeta_l,
-                                            double scale_l, std::uint32_t g0, int lane,
-                                            ItemRows<FP32>& rows) {
-    using Real = real_t<FP32>;
-    const int q = lane >> 4, r = lane & 15;
-    double z[4];
-    right_group<KB, FP32, WIDE>(stage, xs, rmeta_l, g0, lane, z);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const std::uint32_t c = g0 + 4u * std::uint32_t(q) + std::uint32_t(t);  // <= 63
-        const double sc = __shfl(scale_l, int(c), 64);
-        if (r < KB) zt[(4 * q + t) * KB + r] = c < f.count ? Real(sc * z[t]) : Real(0);
-    }
-    wave_lds_sync();
-    left_group<KB, FP32, WIDE>(lstage, zt, rmeta_l, g0, lane, rows);
-    wave_lds_sync();  // the next group rewrites zt
-}
-
-// kRight components [g0, g0 + 16): right-factor sums into the workspace rows
-// of the block's column tile.
-template <int KB, bool FP32, bool WIDE>
-__device__ __forceinline__ void right_rows(const MemberView& M, const Item& f,
-                                           const std::uint32_t* stage, const double* xs,
-                                           std::uint32_t rmeta_l, std::uint32_t g0, int lane,
-                                           int p0, int kk) {
-    const int q = lane >> 4, r = lane & 15;
-    double z[4];
-    right_group<KB, FP32, WIDE>(stage, xs, rmeta_l, g0, lane, z);
-    if (r < kk) {
-        double* out = M.zp + (std::uint64_t(f.zrow) + g0 + 4u * std::uint32_t(q)) *
-                                 std::uint64_t(M.k) + p0 + r;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (g0 + 4u * std::uint32_t(q) + std::uint32_t(t) < f.count)
-                out[std::uint64_t(t) * std::uint64_t(M.k)] = z[t];
-    }
-}
-
 // Component widths and scale products of the staged item (lanes =
 // components), read back from LDS after the staging.
 struct Staged {
@@ -801,60 +447,6 @@ template <int KB>
 __device__ __forceinline__ Staged staged_meta(const std::uint8_t* wave_lds, int lane) {
     return Staged{reinterpret_cast<const std::uint32_t*>(wave_lds + wave_lds_meta<KB>() + 512)[lane],
                   reinterpret_cast<const double*>(wave_lds + wave_lds_meta<KB>())[lane]};
-}
-
-// A kFused, kRight (pass 1) or kLeft (pass 2) item, 16 components at a
-// time.  Row contributions are formed on the matrix cores (in FP32 for the
-// codes in FP32 mode) and added to acc (lanes = rows) in FP64.
-template <int KB, bool FP32, int PASS, bool WIDE>
-__device__ __forceinline__ void lowrank_item(const MemberView& M, const Item& f,
-                                             const Staged& st, int lane, int p0, int kk,
-                                             std::uint32_t R, std::uint8_t* wave_lds,
-                                             double (&acc)[KB]) {
-    using Real = real_t<FP32>;
-    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wave_lds + kLdsHead);
-    double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
-    Real* zt = reinterpret_cast<Real*>(wave_lds + kLdsHead + kStageBytes + kTile * KB * 8);
-    const bool comp = std::uint32_t(lane) < f.count;
-    const std::uint32_t bits_l = st.bits_l;
-    const double scale_l = st.scale_l;
-    ItemRows<FP32> rows;
-    if (PASS == 2) {
-        // kLeft. z of each component: its column tiles' partial sums in order
-        double z[KB];
-#pragma unroll
-        for (int p = 0; p < KB; ++p) z[p] = 0.0;
-        const std::uint32_t nseg = f.nseg;
-        if (comp && (lane + 1) * nseg <= 64u) {
-            for (std::uint32_t sg = 0; sg < nseg; ++sg) {
-#pragma unroll
-                for (int p = 0; p < KB; ++p) z[p] += xs[(lane * nseg + sg) * KB + p];
-            }
-        }
-        std::uint32_t total;
-        const std::uint32_t lmeta_l = side_meta(comp, bits_l, lane, total);
-        for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16) {
-            if (std::uint32_t(lane) - g0 < 16u) {
-#pragma unroll
-                for (int p = 0; p < KB; ++p) zt[(std::uint32_t(lane) - g0) * KB + p] = Real(scale_l * z[p]);
-            }
-            wave_lds_sync();
-            left_group<KB, FP32, WIDE>(stage, zt, lmeta_l, g0, lane, rows);
-            wave_lds_sync();  // the next group rewrites zt
-        }
-    } else {
-        std::uint32_t total;
-        const std::uint32_t rmeta_l = side_meta(comp, bits_l, lane, total);
-        if (f.kind == kRight) {
-            for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
-                right_rows<KB, FP32, WIDE>(M, f, stage, xs, rmeta_l, g0, lane, p0, kk);
-            return;
-        }
-        const std::uint32_t* lstage = stage + side_words(total);
-        for (std::uint32_t g0 = 0; g0 < f.count; g0 += 16)
-            fused_group<KB, FP32, WIDE>(f, stage, lstage, xs, zt, rmeta_l, scale_l, g0, lane, rows);
-    }
-    add_item_rows<KB, FP32>(rows, xs, lane, acc);
 }
 
 // Narrow passes: one staged item on the vector ALUs.  Row contributions are
@@ -953,28 +545,14 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
     }
 }
 
-// Decodes and applies one staged item.  Pass 1 holds kDense, kFused and
-// kRight items, pass 2 kLeft items only.  Narrow passes run on the vector
-// ALUs, wide ones on the matrix cores; in FP32 mode on the matrix cores an
-// item with a code wider than 24 bits takes the split path throughout.
+// Decodes and applies one staged item on the vector ALUs.  Pass 1 holds
+// kDense, kFused and kRight items, pass 2 kLeft items only.
 template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
                                              const Staged& st, int lane, int p0, int kk,
                                              std::uint32_t R, std::uint8_t* wave_lds,
                                              double (&acc)[KB]) {
-    if constexpr (KB < kMatrixMinKB) {
-        valu_item<KB, FP32, PASS>(M, f, st, lane, p0, kk, R, wave_lds, acc);
-    } else if (PASS == 1 && f.kind == kDense) {
-        double* xs = reinterpret_cast<double*>(wave_lds + kLdsHead + kStageBytes);
-        ItemRows<false> rows;
-        dense_rows<KB>(reinterpret_cast<const double*>(wave_lds + kLdsHead), xs, R,
-                       min(f.count, std::uint32_t(kDenseColumns)), lane, rows);
-        add_item_rows<KB, false>(rows, xs, lane, acc);
-    } else if (FP32 && __ballot(std::uint32_t(lane) < f.count && st.bits_l > 24u) != 0) {
-        lowrank_item<KB, FP32, PASS, FP32>(M, f, st, lane, p0, kk, R, wave_lds, acc);
-    } else {
-        lowrank_item<KB, FP32, PASS, false>(M, f, st, lane, p0, kk, R, wave_lds, acc);
-    }
+    valu_item<KB, FP32, PASS>(M, f, st, lane, p0, kk, R, wave_lds, acc);
 }
 
 // One wave's items: [b1, e1) followed by [b2, e2).  Row contributions
@@ -1023,13 +601,9 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
                 fa = fc;
                 fc = slot(s + 4);
             }
-            // Narrow passes take no exit here: an odd count runs one empty
-            // item, whose loads all read zero, so that both steps always run
-            // and the waits for loads in flight stay exact across the loop.
-            // Wide passes exit (one path less keeps their registers in bounds).
-            if constexpr (KB >= kMatrixMinKB) {
-                if (s + 1 >= n) break;
-            }
+            // No exit here: an odd count runs one empty item, whose loads all
+            // read zero, so that both steps always run and the waits for
+            // loads in flight stay exact across the loop.
             {
                 stage_item<KB>(fb, FB, lane, wave_lds);
                 fetch<KB>(M, fd, lane, p0, kk, FB);
@@ -1373,20 +947,64 @@ MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, 
 // Event slots of the next profiled apply, or null when not profiling.
 hipEvent_t* profile_slot(const DeviceStore& s) {
     auto& p = s.profile;
+    std::lock_guard<std::mutex> guard(p.lock);
     if (!p.armed || 3 * (p.used + 1) > p.events.size()) return nullptr;
     return reinterpret_cast<hipEvent_t*>(p.events.data() + 3 * p.used++);
 }
 
-// Waves per workgroup: fewer waves with more registers each as the
-// right-hand sides per pass grow (the wave lists are cut for 16 waves; a
-// wave of a smaller workgroup takes 16/W consecutive lists).
 // Right-hand sides per pass for an apply of k: the tuned choice when one
-// was set for this k (qh_store_set_rhs_block), else k rounded up to a power
-// of two, at most 16.
-int default_rhs_block(std::int64_t k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 16; }
+// was set for this k (qh_store_set_rhs_block), else 1, 2 or 4 on the vector
+// ALUs for k <= 4 and 16, 32 or 64 on the matrix cores above.
+int default_rhs_block(std::int64_t k) {
+    return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
+}
 int rhs_block(const DeviceStore& s, std::int64_t k) {
+    std::shared_lock<std::shared_mutex> guard(s.rhs_lock);
     const auto it = s.rhs_block.find(k);
     return it == s.rhs_block.end() ? default_rhs_block(k) : it->second;
+}
+bool is_wide(int kb) { return kb >= 16; }
+
+WideView wide_view(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
+                   std::int64_t ldy, void* zw) {
+    WideView V{};
+    V.perm = s.perm;
+    V.words = s.words;
+    V.widths = s.widths;
+    V.cscale = s.cscale;
+    V.items = s.witems;
+    V.rruns = s.wruns_r;
+    V.lruns = s.wruns_l;
+    V.sums = s.wsums;
+    V.zsums = s.wzsums;
+    V.ctiles = s.ctiles;
+    V.tiles = s.tiles;
+    V.x = x;
+    V.y = y;
+    V.zw = zw;
+    V.ldx = ldx;
+    V.ldy = ldy;
+    V.k = k;
+    V.z_rows = s.wz_rows;
+    return V;
+}
+
+WideCounts wide_counts(const DeviceStore& s) {
+    WideCounts c;
+    c.right_groups = s.n_wr_groups;
+    c.left_groups = s.n_wl_groups;
+    c.sums = s.n_wsums;
+    c.zsums = s.n_wzsums;
+    return c;
+}
+
+// Workspace of the narrow passes (partial sums, eight row sets, x in
+// clustered order) and of the wide passes (partial sums, row sets).
+std::uint64_t narrow_workspace(const DeviceStore& s, std::int64_t k) {
+    return (s.z_rows + (kRowSets + 1) * s.order) * std::uint64_t(k) * sizeof(double);
+}
+std::uint64_t wide_workspace(const DeviceStore& s, std::int64_t k) {
+    return (s.wz_rows + std::uint64_t(s.wsets) * kTile) * std::uint64_t(k) * sizeof(double);
 }
 
 template <int KB, bool FP32>
@@ -1435,21 +1053,17 @@ void dispatch_batch_class(const BatchClass& c, const double* x, double* y, doubl
     switch (c.kb) {
         case 1: return launch_batch_class<1, FP32>(c, x, y, zp, st);
         case 2: return launch_batch_class<2, FP32>(c, x, y, zp, st);
-        case 4: return launch_batch_class<4, FP32>(c, x, y, zp, st);
-        case 8: return launch_batch_class<8, FP32>(c, x, y, zp, st);
-        default: return launch_batch_class<16, FP32>(c, x, y, zp, st);
+        default: return launch_batch_class<4, FP32>(c, x, y, zp, st);
     }
 }
 
 template <bool FP32>
-void dispatch_rhs(const DeviceStore& s, const double* x, std::int64_t ldx, int k, double* y,
-                  std::int64_t ldy, double* zp, hipStream_t st) {
-    switch (rhs_block(s, k)) {
+void dispatch_narrow(const DeviceStore& s, int kb, const double* x, std::int64_t ldx, int k,
+                     double* y, std::int64_t ldy, double* zp, hipStream_t st) {
+    switch (kb) {
         case 1: return launch_apply<1, FP32>(s, x, ldx, k, y, ldy, zp, st);
         case 2: return launch_apply<2, FP32>(s, x, ldx, k, y, ldy, zp, st);
-        case 4: return launch_apply<4, FP32>(s, x, ldx, k, y, ldy, zp, st);
-        case 8: return launch_apply<8, FP32>(s, x, ldx, k, y, ldy, zp, st);
-        default: return launch_apply<16, FP32>(s, x, ldx, k, y, ldy, zp, st);
+        default: return launch_apply<4, FP32>(s, x, ldx, k, y, ldy, zp, st);
     }
 }
 
@@ -1515,6 +1129,173 @@ void split_runs(const std::vector<std::uint64_t>& cost, std::uint32_t parts, std
         out[w] = base + e;
     }
     out[parts] = base + std::uint32_t(cost.size());
+}
+
+// Components of an item wider than 24 bits (split in FP32 mode).
+std::uint32_t wide_components(const ItemDev& it, const std::vector<std::int32_t>& widths) {
+    std::uint32_t w = 0;
+    for (std::uint32_t j = 0; j < it.count; ++j) w += widths[it.comp0 + j] > 24 ? 1u : 0u;
+    return w;
+}
+
+// Relative time of a wide item (matrix-core passes): per 16 virtual rows of
+// a right side block (16 MFMAs per 16 right-hand sides), per 4 of a left
+// side block (one MFMA, plus a partial-sum load per column tile), a
+// diagonal item as two FP64 steps.
+std::uint64_t wide_right_cost(const ItemDev& it, const std::vector<std::int32_t>& widths) {
+    const std::uint64_t nv = it.count + wide_components(it, widths);
+    return 6 + 16 * ((nv + 15) / 16);
+}
+std::uint64_t wide_left_cost(const ItemDev& it, const std::vector<std::int32_t>& widths) {
+    if (it.kind == kDense) return 10;
+    const std::uint64_t nv = it.count + wide_components(it, widths);
+    return 8 + 4 * ((nv + 3) / 4);
+}
+
+// The wide passes' tables.  Pass R: every right side block (kRight items,
+// and the right halves of kFused items, which get partial-sum rows of their
+// own after the narrow passes' rows) grouped by column tile; each tile's
+// list is cut into workgroups of similar cost (about four per CU over the
+// whole store) and each workgroup's share into kWideWaves wave runs.
+// Pass L: each row tile's kDense items, the left halves of its kFused items
+// and its kLeft items, cut into runs of similar cost (one workgroup each);
+// a tile with several runs gets one row set per run.
+void build_wide(Plan& plan, const std::vector<ItemDev>& right_items,
+                const std::vector<std::vector<ItemDev>>& pass1,
+                const std::vector<std::vector<ItemDev>>& pass2,
+                const std::vector<std::pair<std::uint64_t, std::uint64_t>>& row_tiles,
+                const std::vector<std::pair<std::uint64_t, std::uint64_t>>& col_tiles) {
+    const std::vector<std::int32_t>& widths = plan.widths;
+    const std::size_t nct = col_tiles.size(), nrt = row_tiles.size();
+    auto col_tile_of = [&](std::uint32_t col) {
+        const std::size_t t = first_tile(col_tiles, col);
+        if (t >= nct || col_tiles[t].first != col)
+            throw std::logic_error("device plan: item column is not the start of a column tile");
+        return t;
+    };
+    // quads of the first side block of a kFused item (its right block)
+    auto right_quads = [&](const ItemDev& it) {
+        std::uint64_t bits = 0;
+        for (std::uint32_t j = 0; j < it.count; ++j) bits += std::uint64_t(widths[it.comp0 + j]);
+        return std::uint32_t(16 * ((bits + 31) / 32));
+    };
+    std::vector<std::vector<ItemDev>> vitems(nct), uitems(nrt);
+    // Rows of partial sums: the narrow passes' per-component rows of kRight
+    // items first (z_rows), then rows in virtual-row order, count + (wide
+    // components) per item, the most either precision uses.
+    std::uint64_t wz = plan.z_rows;
+    auto rows_for = [&](const ItemDev& it) {
+        const std::uint64_t r = wz;
+        wz += it.count + wide_components(it, widths);
+        require(wz < (std::uint64_t{1} << 31), "store too large for one device plan");
+        return std::uint32_t(r);
+    };
+    std::map<std::tuple<std::uint32_t, std::uint32_t, std::uint32_t, std::uint32_t>, std::uint32_t>
+        zsum_rows;  // (zrow, zstride, nseg, count) of a kLeft range -> its rows
+    for (ItemDev it : right_items) {
+        it.nseg = 0;  // partial sums per component
+        vitems[col_tile_of(it.col)].push_back(it);
+    }
+    for (std::size_t t = 0; t < nrt; ++t) {
+        for (const auto& it : pass1[t]) {
+            if (it.kind == kDense) {
+                uitems[t].push_back(it);
+                continue;
+            }
+            const std::uint32_t rq = right_quads(it);
+            if (rq == 0 || rq >= it.quads) throw std::logic_error("device plan: fused item sides");
+            const std::uint32_t rows = rows_for(it);
+            ItemDev v = it;
+            v.kind = kRight;
+            v.quads = std::uint16_t(rq);
+            v.zrow = rows;
+            v.nseg = 1;  // partial sums in virtual-row order
+            vitems[col_tile_of(it.col)].push_back(v);
+            ItemDev u = it;
+            u.kind = kLeft;
+            u.quad = it.quad + rq;
+            u.quads = std::uint16_t(it.quads - rq);
+            u.zrow = rows;
+            u.zstride = it.count + wide_components(it, widths);  // rows it may read
+            u.nseg = 1;
+            u.col = u.cols = 0;
+            uitems[t].push_back(u);
+        }
+        for (ItemDev it : pass2[t]) {
+            // the column tiles' partial sums of this component range are
+            // summed once between the passes, into rows in virtual-row order
+            const auto key = std::make_tuple(it.zrow, it.zstride, it.nseg, std::uint32_t(it.count));
+            auto found = zsum_rows.find(key);
+            if (found == zsum_rows.end()) {
+                const std::uint32_t rows = rows_for(it);
+                plan.wzsums.push_back(ZSum{it.zrow, it.zstride, it.nseg, it.count, rows, it.comp0, 0, 0});
+                found = zsum_rows.emplace(key, rows).first;
+            }
+            it.zrow = found->second;
+            it.zstride = it.count + wide_components(it, widths);
+            it.nseg = 1;
+            uitems[t].push_back(it);
+        }
+    }
+    plan.wz_rows = wz;
+    plan.ctiles.resize(nct);
+    for (std::size_t c = 0; c < nct; ++c)
+        plan.ctiles[c] = Span{std::uint32_t(col_tiles[c].first),
+                              std::uint32_t(col_tiles[c].second - col_tiles[c].first)};
+
+    constexpr std::uint64_t kTargetGroups = 1024;  // about four workgroups per CU
+    auto& out = plan.witems;
+    // pass R
+    std::uint64_t rtotal = 0;
+    std::vector<std::vector<std::uint64_t>> rc(nct);
+    for (std::size_t c = 0; c < nct; ++c)
+        for (const auto& it : vitems[c]) {
+            rc[c].push_back(wide_right_cost(it, widths));
+            rtotal += rc[c].back();
+        }
+    const std::uint64_t runit = std::max<std::uint64_t>(1, rtotal / std::max<std::uint64_t>(kTargetGroups, nct));
+    for (std::size_t c = 0; c < nct; ++c) {
+        if (vitems[c].empty()) continue;
+        std::uint64_t cost = 0;
+        for (auto v : rc[c]) cost += v;
+        const std::uint64_t cap = std::max<std::uint64_t>(1, (vitems[c].size() + 1) / 2);
+        const std::uint32_t groups =
+            std::uint32_t(std::clamp<std::uint64_t>((cost + runit / 2) / runit, 1, std::min<std::uint64_t>(cap, 64)));
+        std::vector<std::uint32_t> cut(groups * kWideWaves + 1);
+        split_runs(rc[c], groups * kWideWaves, std::uint32_t(out.size()), cut.data());
+        out.insert(out.end(), vitems[c].begin(), vitems[c].end());
+        for (std::uint32_t w = 0; w < groups * kWideWaves; ++w)
+            plan.wruns_r.push_back(WideRun{std::uint32_t(c), cut[w], cut[w + 1], 0});
+    }
+    // pass L
+    std::uint64_t ltotal = 0;
+    std::vector<std::vector<std::uint64_t>> lc(nrt);
+    for (std::size_t t = 0; t < nrt; ++t)
+        for (const auto& it : uitems[t]) {
+            lc[t].push_back(wide_left_cost(it, widths));
+            ltotal += lc[t].back();
+        }
+    const std::uint64_t lunit = std::max<std::uint64_t>(1, ltotal / std::max<std::uint64_t>(kTargetGroups, nrt));
+    for (std::size_t t = 0; t < nrt; ++t) {
+        std::uint64_t cost = 0;
+        for (auto v : lc[t]) cost += v;
+        const std::uint64_t cap = std::max<std::uint64_t>(1, std::min<std::uint64_t>(8, uitems[t].size()));
+        const std::uint32_t runs =
+            std::uint32_t(std::clamp<std::uint64_t>((cost + lunit / 2) / lunit, 1, cap));
+        std::vector<std::uint32_t> cut(runs + 1);
+        split_runs(lc[t], runs, std::uint32_t(out.size()), cut.data());
+        out.insert(out.end(), uitems[t].begin(), uitems[t].end());
+        if (runs == 1) {
+            plan.wruns_l.push_back(WideRun{std::uint32_t(t), cut[0], cut[1], kNoSet});
+            continue;
+        }
+        const std::uint32_t set0 = plan.wsets;
+        for (std::uint32_t s = 0; s < runs; ++s)
+            plan.wruns_l.push_back(WideRun{std::uint32_t(t), cut[s], cut[s + 1], set0 + s});
+        plan.wsums.push_back(WideSum{std::uint32_t(t), set0, runs, 0});
+        plan.wsets += runs;
+    }
+    require(out.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
 }
 
 }  // namespace
@@ -1875,6 +1656,7 @@ Plan build_plan(const qh_store_desc& d) {
         T.rsplit[w + 1] = next;
     }
     plan.z_rows = z_rows;
+    build_wide(plan, right_items, pass1, pass2, row_tiles, col_tiles);
 
     qh_store_stats& st = plan.stats;
     st.order = n;
@@ -1947,6 +1729,12 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
         s->cscale = upload(plan.cscale, bytes);
         s->items = upload(plan.items, bytes);
         s->tiles = upload(plan.tiles, bytes);
+        s->witems = upload(plan.witems, bytes);
+        s->wruns_r = upload(plan.wruns_r, bytes);
+        s->wruns_l = upload(plan.wruns_l, bytes);
+        s->wsums = upload(plan.wsums, bytes);
+        s->wzsums = upload(plan.wzsums, bytes);
+        s->ctiles = upload(plan.ctiles, bytes);
         s->device_bytes = bytes;
     } catch (...) {
         destroy_device_store(s);
@@ -1957,6 +1745,12 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     s->n_tiles = std::uint32_t(plan.tiles.size());
     s->n_items = std::uint32_t(plan.items.size());
     s->z_rows = plan.z_rows;
+    s->n_wr_groups = std::uint32_t(plan.wruns_r.size() / kWideWaves);
+    s->n_wl_groups = std::uint32_t(plan.wruns_l.size());
+    s->n_wsums = std::uint32_t(plan.wsums.size());
+    s->n_wzsums = std::uint32_t(plan.wzsums.size());
+    s->wsets = plan.wsets;
+    s->wz_rows = plan.wz_rows;
     s->block_to_lowrank = std::move(plan.block_to_lowrank);
     s->left_chunks = std::move(plan.left_chunks);
     s->right_chunks = std::move(plan.right_chunks);
@@ -2121,8 +1915,223 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
         }
 }
 
+// Virtual rows of components [comp0, comp0 + count) in the order the wide
+// kernels use (build_vrows in wide_passes.hip): FP64, one per component in
+// order; FP32 split, the components wider than 24 bits first, each as a
+// high then a low row, then the others in order.  Entries: component j and
+// part (0 whole, 1 high, 2 low).
+std::vector<std::pair<std::uint32_t, int>> vrow_list(const std::vector<std::int32_t>& widths,
+                                                     std::uint32_t comp0, std::uint32_t count,
+                                                     bool fp32) {
+    std::vector<std::pair<std::uint32_t, int>> out;
+    if (fp32)
+        for (std::uint32_t j = 0; j < count; ++j)
+            if (widths.at(comp0 + j) > 24) {
+                out.push_back({j, 1});
+                out.push_back({j, 2});
+            }
+    for (std::uint32_t j = 0; j < count; ++j)
+        if (!fp32 || widths.at(comp0 + j) <= 24) out.push_back({j, 0});
+    return out;
+}
+
+// The wide passes' tables checked on the host: replays pass R, the
+// partial-sum reductions, pass L and the row-set sums with the kernels'
+// index arithmetic for one precision's virtual-row order, every access
+// bounds-checked, and the structure they rely on (each item in exactly one
+// run, a pass-R workgroup's waves on one column tile, codes inside their
+// side block, partial sums written before they are read).  y = H x in FP64.
+void emulate_wide(const Plan& pl, const double* x, std::int64_t k, double* y, bool fp32) {
+    auto check = [](bool ok, const char* what) {
+        if (!ok) throw std::out_of_range(what);
+    };
+    const std::uint64_t n = pl.order, K = std::uint64_t(k);
+    check(pl.wruns_r.size() % kWideWaves == 0, "pass R runs are not whole workgroups");
+    std::vector<int> seen(pl.witems.size(), 0);
+    std::vector<double> z(pl.wz_rows * K, 0.0);
+    std::vector<char> written(pl.wz_rows, 0);
+    auto put = [&](std::uint64_t row, std::uint64_t p, double v) {
+        check(row < pl.wz_rows, "partial-sum row out of range");
+        z[row * K + p] = v;
+        written[row] = 1;
+    };
+    // Code of lane L ending at bit E (width b) of an item's side block, as
+    // the kernels read it (two words, the one past the block reads as zero).
+    auto code_at = [&](const ItemDev& it, std::uint32_t L, std::uint32_t E, std::uint32_t b,
+                       bool swz) {
+        const std::uint64_t nw = it.quads / 16u;
+        check(E >= b && E <= 32 * nw, "code outside its side block");
+        const std::uint64_t kk = E >> 5, sh = E & 31;
+        auto word = [&](std::uint64_t w) -> std::uint64_t {
+            if (w >= nw) return 0;  // the kernels read a zero or slack row, shifted out
+            return at(pl.words, 4ull * it.quad + 64 * w + lane_slot(swz, w, L));
+        };
+        const std::uint64_t pair = (word(kk) << 32) | (kk ? word(kk - 1) : 0);
+        const std::uint64_t field = (pair >> sh) & 0xFFFFFFFFull;  // the 32 bits ending at E
+        const std::int32_t v = std::int32_t(std::uint32_t(field));
+        return double(v >> (32 - b));
+    };
+    auto item_ok = [&](const ItemDev& it) {
+        check(it.quads <= kItemQuads && it.quads % 16 == 0, "item size");
+        check(4ull * (std::uint64_t(it.quad) + it.quads) <= pl.words.size(), "item data range");
+        check(it.count >= 1 && it.count <= kItemComponents, "item component count");
+        check(std::uint64_t(it.comp0) + it.count <= pl.widths.size(), "item components");
+    };
+    auto end_bits = [&](const ItemDev& it) {
+        std::vector<std::uint32_t> E(it.count);
+        std::uint32_t e = 0;
+        for (std::uint32_t j = 0; j < it.count; ++j) E[j] = e += std::uint32_t(pl.widths[it.comp0 + j]);
+        return E;
+    };
+    // pass R
+    for (std::size_t g = 0; g < pl.wruns_r.size() / kWideWaves; ++g) {
+        const std::uint32_t tile = pl.wruns_r[g * kWideWaves].tile;
+        const Span& span = at(pl.ctiles, tile);
+        check(span.count >= 1 && span.count <= std::uint32_t(kTile) && span.first + span.count <= n,
+              "column tile");
+        for (int w = 0; w < kWideWaves; ++w) {
+            const WideRun& run = pl.wruns_r[g * kWideWaves + w];
+            check(run.tile == tile, "a pass-R workgroup spans two column tiles");
+            check(run.begin <= run.end && run.end <= pl.witems.size(), "pass R run range");
+            for (std::uint32_t i = run.begin; i < run.end; ++i) {
+                const ItemDev& it = pl.witems[i];
+                ++seen[i];
+                item_ok(it);
+                check(it.kind == kRight && it.col == span.first && it.cols <= span.count &&
+                          it.nseg <= 1,
+                      "pass R item is not a right side block of its column tile");
+                const auto E = end_bits(it);
+                std::vector<double> zs(std::uint64_t(it.count) * K);
+                for (std::uint32_t j = 0; j < it.count; ++j) {
+                    const std::uint32_t b = std::uint32_t(pl.widths[it.comp0 + j]);
+                    const double sc = pl.cscale[it.comp0 + j];
+                    for (std::uint64_t p = 0; p < K; ++p) {
+                        double sum = 0.0;
+                        for (std::uint32_t c = 0; c < span.count; ++c) {
+                            const double cv = c < it.cols ? code_at(it, c, E[j], b, true) : 0.0;
+                            sum += cv * x[std::uint64_t(at(pl.perm, span.first + c)) * K + p];
+                        }
+                        zs[j * K + p] = sc * sum;
+                    }
+                }
+                if (it.nseg == 0) {  // per component
+                    for (std::uint32_t j = 0; j < it.count; ++j)
+                        for (std::uint64_t p = 0; p < K; ++p)
+                            put(std::uint64_t(it.zrow) + j, p, zs[j * K + p]);
+                } else {  // in virtual-row order
+                    const auto vl = vrow_list(pl.widths, it.comp0, it.count, fp32);
+                    for (std::size_t v = 0; v < vl.size(); ++v)
+                        for (std::uint64_t p = 0; p < K; ++p)
+                            put(std::uint64_t(it.zrow) + v, p, zs[vl[v].first * K + p]);
+                }
+            }
+        }
+    }
+    // partial sums over a block's column tiles, into rows in virtual-row order
+    for (const ZSum& zr : pl.wzsums) {
+        check(zr.nseg >= 1 && zr.count >= 1, "partial-sum reduction shape");
+        const auto vl = vrow_list(pl.widths, zr.comp0, zr.count, fp32);
+        for (std::size_t v = 0; v < vl.size(); ++v)
+            for (std::uint64_t p = 0; p < K; ++p) {
+                double s = 0.0;
+                for (std::uint32_t sg = 0; sg < zr.nseg; ++sg) {
+                    const std::uint64_t row =
+                        std::uint64_t(zr.zrow) + std::uint64_t(sg) * zr.zstride + vl[v].first;
+                    check(row < pl.z_rows && written[row], "reduction reads an unwritten partial sum");
+                    s += z[row * K + p];
+                }
+                check(std::uint64_t(zr.vzrow) + v >= pl.z_rows, "reduction overwrites partial sums");
+                put(std::uint64_t(zr.vzrow) + v, p, s);
+            }
+    }
+    // pass L
+    std::vector<double> sets(std::uint64_t(pl.wsets) * kTile * K, 0.0);
+    std::vector<int> tile_runs(pl.tiles.size(), 0), direct(pl.tiles.size(), 0);
+    for (const WideRun& run : pl.wruns_l) {
+        const TileDev& T = at(pl.tiles, run.tile);
+        const std::uint32_t R = T.rows;
+        check(R >= 1 && R <= std::uint32_t(kTile), "tile rows");
+        check(run.begin <= run.end && run.end <= pl.witems.size(), "pass L run range");
+        ++tile_runs[run.tile];
+        std::vector<double> acc(std::uint64_t(kTile) * K, 0.0);
+        for (std::uint32_t i = run.begin; i < run.end; ++i) {
+            const ItemDev& it = pl.witems[i];
+            ++seen[i];
+            if (it.kind == kDense) {
+                check(it.count >= 1 && it.count <= kDenseColumns, "dense item columns");
+                check(std::uint64_t(it.col) + it.count <= n, "dense item column range");
+                check(4ull * it.quad + 2ull * it.count * R <= pl.words.size(), "dense item data");
+                for (std::uint32_t c = 0; c < it.count; ++c)
+                    for (std::uint32_t r = 0; r < R; ++r) {
+                        const std::uint64_t wv = 4ull * it.quad + 2ull * (std::uint64_t(c) * R + r);
+                        const std::uint64_t bitsv = std::uint64_t(pl.words[wv]) |
+                                                    (std::uint64_t(pl.words[wv + 1]) << 32);
+                        double v;
+                        std::memcpy(&v, &bitsv, 8);
+                        for (std::uint64_t p = 0; p < K; ++p)
+                            acc[r * K + p] += v * x[std::uint64_t(at(pl.perm, it.col + c)) * K + p];
+                    }
+                continue;
+            }
+            item_ok(it);
+            check(it.kind == kLeft && it.nseg == 1, "pass L item kind");
+            check(it.quads / 16u <= 16u, "left side block rows");
+            const auto E = end_bits(it);
+            const auto vl = vrow_list(pl.widths, it.comp0, it.count, fp32);
+            check(vl.size() <= it.zstride && it.zstride <= 2u * it.count, "virtual rows of the item");
+            check(std::uint64_t(it.zrow) + it.zstride <= pl.wz_rows, "partial-sum rows of the item");
+            for (std::size_t v = 0; v < vl.size(); ++v) {
+                const std::uint32_t j = vl[v].first;
+                const std::uint32_t b = std::uint32_t(pl.widths[it.comp0 + j]);
+                const std::uint64_t row = std::uint64_t(it.zrow) + v;
+                check(written[row], "pass L reads an unwritten partial sum");
+                for (std::uint32_t r = 0; r < R; ++r) {
+                    // the part of the code this row carries, the high part with its 2^16
+                    const double cv = code_at(it, r, E[j], b, false);
+                    double a = cv;
+                    if (vl[v].second != 0) {
+                        const std::int64_t c = std::int64_t(cv);
+                        const std::int64_t lo = std::int64_t(std::int16_t(std::uint16_t(c & 0xFFFF)));
+                        a = vl[v].second == 2 ? double(lo) : double(c - lo);
+                    }
+                    for (std::uint64_t p = 0; p < K; ++p) acc[r * K + p] += a * z[row * K + p];
+                }
+            }
+        }
+        for (std::uint32_t r = 0; r < R; ++r)
+            for (std::uint64_t p = 0; p < K; ++p) {
+                if (run.set == kNoSet) {
+                    ++direct[run.tile];
+                    y[std::uint64_t(at(pl.perm, T.row0 + r)) * K + p] = acc[r * K + p];
+                } else {
+                    check(run.set < pl.wsets, "row set index");
+                    sets[(std::uint64_t(run.set) * kTile + r) * K + p] = acc[r * K + p];
+                }
+            }
+    }
+    for (std::size_t t = 0; t < pl.tiles.size(); ++t)
+        check(tile_runs[t] >= 1 && (direct[t] == 0 || tile_runs[t] == 1),
+              "row tile without runs, or several runs writing y");
+    for (const WideSum& ws : pl.wsums) {
+        const TileDev& T = at(pl.tiles, ws.tile);
+        check(direct[ws.tile] == 0 && std::uint64_t(ws.set0) + ws.count <= pl.wsets &&
+                  int(ws.count) == tile_runs[ws.tile],
+              "row-set sum");
+        for (std::uint32_t r = 0; r < T.rows; ++r)
+            for (std::uint64_t p = 0; p < K; ++p) {
+                double v = 0.0;
+                for (std::uint32_t s = 0; s < ws.count; ++s)
+                    v += sets[(std::uint64_t(ws.set0 + s) * kTile + r) * K + p];
+                y[std::uint64_t(at(pl.perm, T.row0 + r)) * K + p] = v;
+            }
+    }
+    for (int c : seen) check(c == 1, "a wide item is not in exactly one run");
+}
+
 void destroy_device_store(DeviceStore* s) {
     if (!s) return;
+    if (s->pins.load() > 0)
+        throw std::invalid_argument("store is in use by a batch: destroy the batch first");
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(s->device);
@@ -2130,16 +2139,32 @@ void destroy_device_store(DeviceStore* s) {
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
                     static_cast<void*>(s->widths), static_cast<void*>(s->cscale),
                     static_cast<void*>(s->items), static_cast<void*>(s->tiles),
-                    static_cast<void*>(s->rec.tasks), static_cast<void*>(s->rec.chunks),
+                    static_cast<void*>(s->witems), static_cast<void*>(s->wruns_r),
+                    static_cast<void*>(s->wruns_l), static_cast<void*>(s->wsums),
+                    static_cast<void*>(s->wzsums),
+                    static_cast<void*>(s->ctiles), static_cast<void*>(s->rec.tasks), static_cast<void*>(s->rec.chunks),
                     static_cast<void*>(s->rec.lscale), static_cast<void*>(s->rec.rscale)})
         if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete s;
 }
 
+namespace {
+// Columns per launch chain of the narrow passes: all k, or the most (a
+// multiple of kb) whose workspace stays addressable by the kernels' 32-bit
+// buffer offsets.
+std::int64_t narrow_columns(const DeviceStore& s, std::int64_t k, int kb) {
+    if (narrow_workspace(s, k) < kOutOfRange) return k;
+    const std::uint64_t per = narrow_workspace(s, 1);
+    const std::int64_t cols = std::int64_t((kOutOfRange - 1) / per) / kb * kb;
+    require(cols >= kb, "store too large for the narrow passes (use 16 or more right-hand sides per pass)");
+    return cols;
+}
+}  // namespace
+
 std::uint64_t apply_workspace_bytes(const DeviceStore& s, std::int64_t k) {
-    // partial sums, the row sets, then x in clustered order
-    return (s.z_rows + (kRowSets + 1) * s.order) * std::uint64_t(k) * sizeof(double);
+    // enough for either path, whichever the right-hand-side block selects
+    return std::max(narrow_workspace(s, narrow_columns(s, k, 4)), wide_workspace(s, k));
 }
 
 void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t k, double* y,
@@ -2151,12 +2176,17 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
     require(x != nullptr && y != nullptr, "null right-hand side or result pointer");
     require(precision == QH_PRECISION_FP64 || precision == QH_PRECISION_FP32SPLIT,
             "unknown precision mode");
-    const std::uint64_t need = apply_workspace_bytes(s, k);
+    const bool fp32 = precision == QH_PRECISION_FP32SPLIT;
+    const int kb = rhs_block(s, k);
+    const bool wide = is_wide(kb);
+    // the narrow passes run in column chunks whose workspace and x rows stay
+    // below their 32-bit buffer offsets; the wide passes address 64-bit
+    const std::int64_t cols = wide ? k : narrow_columns(s, k, kb);
+    const std::uint64_t need = wide ? wide_workspace(s, k) : narrow_workspace(s, cols);
     require(workspace == nullptr || workspace_bytes >= need, "workspace is too small");
-    // the kernels address x and the workspace with 32-bit byte offsets
-    require(((s.order - 1) * std::uint64_t(ldx) + std::uint64_t(k)) * sizeof(double) < kOutOfRange &&
-                need < kOutOfRange,
-            "right-hand sides too large for one apply (split the columns)");
+    require(wide || ((s.order - 1) * std::uint64_t(ldx) + std::uint64_t(cols)) * sizeof(double) <
+                        kOutOfRange,
+            "right-hand-side leading dimension too large for the narrow passes");
     DeviceScope scope(s.device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     double* zp = static_cast<double*>(workspace);
@@ -2166,10 +2196,18 @@ void apply(const DeviceStore& s, const double* x, std::int64_t ldx, std::int64_t
         owned = true;
     }
     try {
-        if (precision == QH_PRECISION_FP64)
-            dispatch_rhs<false>(s, x, ldx, int(k), y, ldy, zp, st);
-        else
-            dispatch_rhs<true>(s, x, ldx, int(k), y, ldy, zp, st);
+        if (wide) {
+            launch_wide(wide_view(s, x, ldx, int(k), y, ldy, zp), wide_counts(s), kb, fp32, st,
+                        profile_slot(s));
+        } else {
+            for (std::int64_t c0 = 0; c0 < k; c0 += cols) {
+                const int kc = int(std::min(cols, k - c0));
+                if (fp32)
+                    dispatch_narrow<true>(s, kb, x + c0, ldx, kc, y + c0, ldy, zp, st);
+                else
+                    dispatch_narrow<false>(s, kb, x + c0, ldx, kc, y + c0, ldy, zp, st);
+            }
+        }
     } catch (...) {
         if (owned) (void)hipFreeAsync(zp, st);
         throw;
@@ -2217,6 +2255,7 @@ void profile_begin(DeviceStore& s, std::uint64_t max_applies) {
     require(max_applies > 0 && max_applies < (std::uint64_t{1} << 20), "invalid profile size");
     DeviceScope scope(s.device);
     auto& p = s.profile;
+    std::lock_guard<std::mutex> guard(p.lock);
     while (p.events.size() < 3 * max_applies) {
         hipEvent_t e;
         hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -2229,6 +2268,7 @@ void profile_begin(DeviceStore& s, std::uint64_t max_applies) {
 qh_profile profile_end(DeviceStore& s) {
     DeviceScope scope(s.device);
     auto& p = s.profile;
+    std::lock_guard<std::mutex> guard(p.lock);
     qh_profile out{};
     out.applies = p.used;
     for (std::uint64_t a = 0; a < p.used; ++a) {
@@ -2278,8 +2318,9 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
 
 void set_rhs_block(DeviceStore& s, std::int64_t k, int kb) {
     require(k > 0, "right-hand-side shape mismatch");
-    require(kb == 0 || kb == 1 || kb == 2 || kb == 4 || kb == 8 || kb == 16,
-            "right-hand sides per pass must be 1, 2, 4, 8 or 16 (0 = default)");
+    require(kb == 0 || kb == 1 || kb == 2 || kb == 4 || kb == 16 || kb == 32 || kb == 64,
+            "right-hand sides per pass must be 1, 2, 4, 16, 32 or 64 (0 = default)");
+    std::unique_lock<std::shared_mutex> guard(s.rhs_lock);
     if (kb == 0) s.rhs_block.erase(k);
     else s.rhs_block[k] = kb;
 }
@@ -2290,28 +2331,39 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
     try {
         b->device = stores[0] ? stores[0]->device : 0;
         std::uint64_t off = 0, zoff = 0;
+        std::vector<int> kbs;
         for (std::uint64_t i = 0; i < count; ++i) {
             require(stores[i] != nullptr, "null store in batch");
             require(stores[i]->device == b->device, "batch members must share one device");
             require(k[i] > 0 && k[i] <= std::int64_t(std::numeric_limits<int>::max()),
                     "right-hand-side shape mismatch");
+            const int kb = rhs_block(*stores[i], k[i]);
+            const std::uint64_t ws =
+                is_wide(kb) ? wide_workspace(*stores[i], k[i]) : narrow_workspace(*stores[i], k[i]);
+            // the narrow passes address a member's x and workspace with 32-bit offsets
+            require(is_wide(kb) || (ws < kOutOfRange &&
+                                    stores[i]->order * std::uint64_t(k[i]) * sizeof(double) < kOutOfRange),
+                    "batch member too large for the narrow passes (use 16 or more right-hand "
+                    "sides per pass)");
+            kbs.push_back(kb);
             b->stores.push_back(stores[i]);
             b->k.push_back(k[i]);
             b->offset.push_back(off);
             b->zoffset.push_back(zoff);
             off += stores[i]->order * std::uint64_t(k[i]);
-            zoff += apply_workspace_bytes(*stores[i], k[i]) / sizeof(double);
+            zoff += ws / sizeof(double);
         }
         b->total = off;
         b->z_total = zoff;
+        for (const DeviceStore* s : b->stores) s->pins.fetch_add(1);
         DeviceScope scope(b->device);
-        for (int kb : {1, 2, 4, 8, 16}) {
+        for (int kb : {1, 2, 4}) {
             BatchClass c;
             c.kb = kb;
             std::vector<MemberView> views;
             std::vector<std::uint32_t> ft{0};
             for (std::uint64_t i = 0; i < count; ++i) {
-                if (rhs_block(*b->stores[i], b->k[i]) != kb) continue;
+                if (kbs[i] != kb) continue;
                 const DeviceStore& s = *b->stores[i];
                 // x, y and the workspace as element offsets from the call's bases
                 views.push_back(member_view(
@@ -2331,6 +2383,42 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
             c.first_tile = upload(ft, bytes);
             b->classes.push_back(c);
         }
+        for (int kc : {16, 32, 64}) {
+            WideBatchClass c;
+            c.kc = kc;
+            std::vector<WideView> views;
+            std::vector<std::uint32_t> fr{0}, fl{0}, fs{0}, fz{0};
+            for (std::uint64_t i = 0; i < count; ++i) {
+                if (kbs[i] != kc) continue;
+                const DeviceStore& s = *b->stores[i];
+                // x and y as offsets in doubles, the workspace in bytes
+                views.push_back(wide_view(
+                    s, reinterpret_cast<const double*>(std::uintptr_t(b->offset[i])), b->k[i],
+                    int(b->k[i]), reinterpret_cast<double*>(std::uintptr_t(b->offset[i])),
+                    b->k[i], reinterpret_cast<void*>(std::uintptr_t(b->zoffset[i] * sizeof(double)))));
+                require(std::uint64_t(fl.back()) + s.n_wl_groups + s.n_wr_groups + s.n_wsums <
+                            (std::uint64_t{1} << 31),
+                        "batch too large for one launch");
+                fr.push_back(fr.back() + s.n_wr_groups);
+                fl.push_back(fl.back() + s.n_wl_groups);
+                fs.push_back(fs.back() + s.n_wsums);
+                fz.push_back(fz.back() + s.n_wzsums);
+                c.max_k = std::max(c.max_k, b->k[i]);
+            }
+            if (views.empty()) continue;
+            c.member_count = views.size();
+            c.right_groups = fr.back();
+            c.left_groups = fl.back();
+            c.sums = fs.back();
+            c.zsums = fz.back();
+            std::uint64_t bytes = 0;
+            c.members = upload(views, bytes);
+            c.first_right = upload(fr, bytes);
+            c.first_left = upload(fl, bytes);
+            c.first_sum = upload(fs, bytes);
+            c.first_zsum = upload(fz, bytes);
+            b->wclasses.push_back(c);
+        }
     } catch (...) {
         destroy_batch(b);
         throw;
@@ -2340,11 +2428,19 @@ Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std
 
 void destroy_batch(Batch* b) {
     if (!b) return;
+    if (b->pins.load() > 0)
+        throw std::invalid_argument("batch is in use by a graph: destroy the graph first");
+    for (const DeviceStore* s : b->stores) s->pins.fetch_sub(1);
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(b->device);
     for (auto& c : b->classes)
         for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_tile)})
+            if (p) (void)hipFree(p);
+    for (auto& c : b->wclasses)
+        for (void* p : {static_cast<void*>(c.members), static_cast<void*>(c.first_right),
+                        static_cast<void*>(c.first_left), static_cast<void*>(c.first_sum),
+                        static_cast<void*>(c.first_zsum)})
             if (p) (void)hipFree(p);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete b;
@@ -2357,8 +2453,6 @@ void batch_apply(const Batch& b, const double* x, double* y, int precision, void
             "unknown precision mode");
     const std::uint64_t need = b.z_total * sizeof(double);
     require(workspace == nullptr || workspace_bytes >= need, "workspace is too small");
-    require(b.total * sizeof(double) < kOutOfRange && need < kOutOfRange,
-            "batch right-hand sides too large for one launch");
     DeviceScope scope(b.device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     double* zp = static_cast<double*>(workspace);
@@ -2367,13 +2461,15 @@ void batch_apply(const Batch& b, const double* x, double* y, int precision, void
         hip_check(hipMallocAsync(reinterpret_cast<void**>(&zp), need, st), "hipMallocAsync");
         owned = true;
     }
+    const bool fp32 = precision == QH_PRECISION_FP32SPLIT;
     try {
         for (const auto& c : b.classes) {
-            if (precision == QH_PRECISION_FP64)
-                dispatch_batch_class<false>(c, x, y, zp, st);
-            else
+            if (fp32)
                 dispatch_batch_class<true>(c, x, y, zp, st);
+            else
+                dispatch_batch_class<false>(c, x, y, zp, st);
         }
+        for (const auto& c : b.wclasses) launch_batch_wide(c, x, y, zp, fp32, st);
     } catch (...) {
         if (owned) (void)hipFreeAsync(zp, st);
         throw;

@@ -39,9 +39,11 @@
 //           through the permutation.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <vector>
 
 #include "../../include/qhmat_hip.h"
@@ -83,6 +85,77 @@ struct TileDev {
     std::uint32_t rsplit[kWaves + 1];
     std::uint32_t split1[kWaves + 1];
     std::uint32_t split2[kWaves + 1];
+};
+
+// ---- wide passes: 16, 32 or 64 right-hand sides per pass on the matrix cores
+//
+// The same side blocks as above, in a second item table (`witems`) ordered
+// for two passes and a combine:
+//  pass R  workgroups of kWideWaves waves own one column tile: x of its
+//          columns is loaded into LDS once, and each wave streams a run of
+//          the tile's right-factor side blocks (every kRight item, and the
+//          right half of every kFused item), decoding each code once for all
+//          the pass's right-hand sides and writing the scaled partial sums
+//          z = cscale * V^T x of the tile to the workspace: per component for
+//          a kRight item, in virtual-row order (the order pass L reads them
+//          in, see wide_passes.hip) for a kFused item's right half;
+//  zsum    one workgroup per (block, component range) of the kLeft items
+//          sums the partial sums of the block's column tiles into rows in
+//          virtual-row order;
+//  pass L  workgroups of kWideWaves waves own a run of one row tile's items
+//          (its diagonal kDense items, kLeft items and the left halves of
+//          its kFused items); wave w forms rows 16 w .. 16 w + 15, summing
+//          each block's product in FP32 (FP32 mode) and the blocks in FP64;
+//          a tile with one run writes y, others write row sets;
+//  combine sums a tile's row sets in a fixed order into y.
+constexpr int kWideWaves = 4;
+constexpr std::uint32_t kNoSet = 0xFFFFFFFFu;
+
+// A run of wide items [begin, end): pass R, one per wave (all waves of a
+// workgroup share `tile`, a column tile); pass L, one per workgroup (`tile`
+// a row tile, `set` its row set or kNoSet when the run writes y).
+struct WideRun {
+    std::uint32_t tile, begin, end, set;
+};
+// Row tile `tile` whose pass-L runs wrote row sets [set0, set0 + count).
+struct WideSum {
+    std::uint32_t tile, set0, count, pad;
+};
+// Column tile: clustered columns [first, first + count).
+struct Span {
+    std::uint32_t first, count;
+};
+// The partial sums of a block's components [comp0, comp0 + count) over its
+// nseg column tiles (rows zrow + s zstride + j), summed in order between the
+// two passes into rows vzrow + v in the components' virtual-row order (the
+// order pass L reads them in).
+struct ZSum {
+    std::uint32_t zrow, zstride, nseg, count, vzrow, comp0, pad0, pad1;
+};
+static_assert(sizeof(ZSum) == 32, "ZSum is 32 bytes");
+
+// Everything the wide passes read, plus one call's operands (x, y and the
+// workspace as offsets from the call's bases in a batch table).
+struct WideView {
+    const std::int32_t* perm;
+    const std::uint32_t* words;
+    const std::int32_t* widths;
+    const double* cscale;
+    const ItemDev* items;
+    const WideRun* rruns;
+    const WideRun* lruns;
+    const WideSum* sums;
+    const ZSum* zsums;
+    const Span* ctiles;
+    const TileDev* tiles;
+    const double* x;
+    double* y;
+    void* zw;  // workspace: z_rows x k partial sums (FP32 in FP32 mode, FP64
+               // otherwise, row stride k), then the row sets: sets x 64 x k FP64
+    std::int64_t ldx, ldy;
+    std::int32_t k;
+    std::uint32_t pad;
+    std::uint64_t z_rows;
 };
 
 // One store's device arrays plus one call's operands: everything a work item
@@ -137,6 +210,9 @@ struct RecTables {
 
 struct DeviceStore {
     int device = 0;
+    // batches whose member tables hold this store's device pointers; the
+    // store refuses destruction while any exist
+    mutable std::atomic<int> pins{0};
     std::uint64_t order = 0;
     // device arrays
     std::int32_t* perm = nullptr;
@@ -145,6 +221,15 @@ struct DeviceStore {
     double* cscale = nullptr;
     ItemDev* items = nullptr;
     TileDev* tiles = nullptr;
+    // wide passes
+    ItemDev* witems = nullptr;
+    WideRun* wruns_r = nullptr;
+    WideRun* wruns_l = nullptr;
+    WideSum* wsums = nullptr;
+    ZSum* wzsums = nullptr;
+    Span* ctiles = nullptr;
+    std::uint32_t n_wr_groups = 0, n_wl_groups = 0, n_wsums = 0, n_wzsums = 0, wsets = 0;
+    std::uint64_t wz_rows = 0;
     // sizes
     std::uint64_t n_words = 0, n_comp = 0;
     std::uint32_t n_tiles = 0, n_items = 0;
@@ -159,6 +244,7 @@ struct DeviceStore {
     // Right-hand sides per pass chosen for a right-hand-side count
     // (qh_store_set_rhs_block, e.g. by the autotuner); others use the default.
     std::map<std::int64_t, int> rhs_block;
+    mutable std::shared_mutex rhs_lock;  // applies read rhs_block, set_rhs_block writes it
     // Reconstruction tables: host copies, device copies made by the first
     // qh_reconstruct.
     struct Rec {
@@ -176,6 +262,7 @@ struct DeviceStore {
         std::vector<void*> events;  // hipEvent_t, 3 per apply
         std::uint64_t used = 0;     // applies recorded
         bool armed = false;
+        std::mutex lock;
     };
     mutable Profile profile;
 };
@@ -195,6 +282,14 @@ struct Plan {
     std::uint64_t z_rows = 0, stored_words = 0;
     qh_store_stats stats{};
     RecTables rec;
+    // wide passes
+    std::vector<ItemDev> witems;
+    std::vector<WideRun> wruns_r, wruns_l;
+    std::vector<WideSum> wsums;
+    std::vector<ZSum> wzsums;
+    std::vector<Span> ctiles;
+    std::uint64_t wz_rows = 0;  // partial-sum rows of the wide passes
+    std::uint32_t wsets = 0;    // row sets of the wide passes
 };
 
 // Throw std::invalid_argument / std::runtime_error like the host store.
@@ -203,10 +298,12 @@ Plan build_plan(const qh_store_desc& desc);
 // with every array access bounds-checked (std::out_of_range), and returns
 // y = H x in FP64.  A check of the layout without a device.
 void emulate_plan(const Plan& plan, const double* x, std::int64_t k, double* y);
+// The same for the wide passes' tables, in one precision's virtual-row order.
+void emulate_wide(const Plan& plan, const double* x, std::int64_t k, double* y, bool fp32);
 DeviceStore* create_device_store(const qh_store_desc& desc, int device);
 
-// One launch pair of a batch: the members whose right-hand-side count rounds
-// up to the same power of two kb (<= 16; larger counts take several passes).
+// One launch chain of a batch on the narrow passes: the members applied with
+// kb (1, 2 or 4) right-hand sides per pass.
 struct BatchClass {
     int kb = 1;
     std::uint64_t member_count = 0;
@@ -216,17 +313,45 @@ struct BatchClass {
     std::uint32_t* first_tile = nullptr;  // device, member_count + 1 prefix counts
 };
 
+// Workgroups of one apply's launches.
+struct WideCounts {
+    std::uint32_t right_groups = 0;  // pass R (kWideWaves runs each)
+    std::uint32_t zsums = 0;         // partial-sum reductions between the passes
+    std::uint32_t left_groups = 0;   // pass L (one run each)
+    std::uint32_t sums = 0;          // row tiles with several pass-L runs
+};
+
+// One class of a batch: the members applied with kc right-hand sides per
+// pass, their tables in device memory (x, y and the workspace as offsets in
+// doubles, doubles and bytes from the call's bases) and the prefix counts
+// of each launch's workgroups.
+struct WideBatchClass {
+    int kc = 16;
+    std::uint64_t member_count = 0;
+    std::int64_t max_k = 1;
+    std::uint32_t right_groups = 0, left_groups = 0, sums = 0, zsums = 0;
+    WideView* members = nullptr;
+    std::uint32_t* first_zsum = nullptr;
+    std::uint32_t* first_right = nullptr;
+    std::uint32_t* first_left = nullptr;
+    std::uint32_t* first_sum = nullptr;
+};
+
 // A fixed set of resident stores applied together.  Member i reads its
 // order_i x k_i right-hand sides at x + offset[i] (row-major, leading
 // dimension k_i), writes y at the same offset, and uses workspace doubles
 // from zoffset[i].
 struct Batch {
     int device = 0;
+    // graphs captured from this batch (they replay its member tables); the
+    // batch refuses destruction while any exist
+    std::atomic<int> pins{0};
     std::vector<const DeviceStore*> stores;
     std::vector<std::int64_t> k;
     std::vector<std::uint64_t> offset, zoffset;
     std::uint64_t total = 0, z_total = 0;
     std::vector<BatchClass> classes;
+    std::vector<WideBatchClass> wclasses;
 };
 
 Batch* create_batch(const DeviceStore* const* stores, const std::int64_t* k, std::uint64_t count);

@@ -50,9 +50,13 @@ def apply(native, x, y=None, *, precision: str = "fp64", stream=None, workspace=
     """y = H x for an FP64 device tensor x of shape (n,) or (n, k).
 
     Rows of x and y must be contiguous (stride 1 along k); any row stride is
-    accepted.  The call is ordered on `stream` (default: the current stream of
-    x's device) and does not synchronise.  `workspace`: optional uint8 device
-    tensor of at least workspace_bytes(native, k).
+    accepted.  The apply is ordered on `stream` (a torch stream or a raw
+    hipStream_t; default: the current stream of x's device) after the work
+    already queued on the current stream, and does not synchronise: the
+    result is ready on `stream`.  Copies and allocations made here are made
+    on `stream`, and tensors it reads or writes that come from elsewhere are
+    recorded on it for the caching allocator.  `workspace`: optional uint8
+    device tensor of at least workspace_bytes(native, k).
     """
     torch = _torch()
     if not isinstance(x, torch.Tensor) or not x.is_cuda:
@@ -69,45 +73,50 @@ def apply(native, x, y=None, *, precision: str = "fp64", stream=None, workspace=
     k = x2.shape[1]
     if k == 0:
         raise ValueError("right-hand-side shape mismatch")
-    if x2.stride(1) != 1 and k > 1:
-        x2 = x2.contiguous()
-    device = x2.device.index if x2.device.index is not None else torch.cuda.current_device()
-    if y is None:
-        y2 = torch.empty((n, k), dtype=torch.float64, device=x2.device)
-    else:
+    y2 = None
+    if y is not None:
         y2 = y.unsqueeze(1) if y.dim() == 1 else y
         if y2.shape != (n, k) or y2.dtype != torch.float64 or y2.device != x2.device:
             raise ValueError("result tensor has the wrong shape, dtype or device")
         if y2.stride(1) != 1 and k > 1:
             raise ValueError("result rows must be contiguous")
-    store = native.device_store(device)
+    device = x2.device.index if x2.device.index is not None else torch.cuda.current_device()
+    current = torch.cuda.current_stream(x2.device)
     if stream is None:
-        stream = torch.cuda.current_stream(x2.device)
-    handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
-    ws_ptr, ws_bytes = None, 0
-    if workspace is None:
-        need = workspace_bytes(native, k, device)
-        if need:
-            with torch.cuda.stream(stream) if hasattr(stream, "cuda_stream") else _null():
+        st = current
+    elif hasattr(stream, "cuda_stream"):
+        st = stream
+    else:
+        st = torch.cuda.ExternalStream(int(stream), device=x2.device)
+    other = st.cuda_stream != current.cuda_stream
+    if other:
+        st.wait_stream(current)  # x and y as the caller's current stream leaves them
+    store = native.device_store(device)
+    with torch.cuda.stream(st):
+        if x2.stride(1) != 1 and k > 1:
+            x2 = x2.contiguous()
+        if y2 is None:
+            y2 = torch.empty((n, k), dtype=torch.float64, device=x2.device)
+        ws_ptr, ws_bytes = None, 0
+        if workspace is None:
+            need = workspace_bytes(native, k, device)
+            if need:
                 workspace = torch.empty(need, dtype=torch.uint8, device=x2.device)
-    if workspace is not None:
-        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
-    ldx = x2.stride(0) if k > 1 else max(x2.stride(0), 1)
-    ldy = y2.stride(0) if k > 1 else max(y2.stride(0), 1)
-    _native.check(_native.lib().qh_apply(
-        store, C.c_void_p(x2.data_ptr()), ldx, k, C.c_void_p(y2.data_ptr()), ldy,
-        precision_code(precision), C.c_void_p(handle), C.c_void_p(ws_ptr), ws_bytes))
+        if workspace is not None:
+            ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+        ldx = x2.stride(0) if k > 1 else max(x2.stride(0), 1)
+        ldy = y2.stride(0) if k > 1 else max(y2.stride(0), 1)
+        _native.check(_native.lib().qh_apply(
+            store, C.c_void_p(x2.data_ptr()), ldx, k, C.c_void_p(y2.data_ptr()), ldy,
+            precision_code(precision), C.c_void_p(st.cuda_stream), C.c_void_p(ws_ptr), ws_bytes))
+    if other:
+        # memory made on another stream, used on st: not reused before st is done
+        for t in (x2, y2, workspace):
+            if t is not None:
+                t.record_stream(st)
     if y is not None:
         return y
     return y2.squeeze(1) if vec else y2
-
-
-class _null:
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        return False
 
 
 def apply_numpy(native, a: np.ndarray, precision: str = "fp64") -> np.ndarray:

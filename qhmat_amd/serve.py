@@ -77,6 +77,7 @@ class Server:
         self._slots = threading.Semaphore(slots)
         self._streams = [torch.cuda.Stream(device=device) for _ in range(slots)]
         self._next_stream = 0
+        self._busy: dict[str, int] = {}  # passes in flight per store
         self._done: queue.Queue = queue.Queue()
         self._running = False
         self.latencies: list[float] = []
@@ -101,9 +102,21 @@ class Server:
         return size
 
     def evict(self, name: str) -> None:
+        """Frees a resident store.  Requests for it still queued fail with
+        KeyError; passes already launched on it finish first."""
         with self._cv:
             res = self._stores.pop(name)
             self._resident_bytes -= res.device_bytes
+            kept = collections.deque()
+            for req in self._queue:
+                if req.store == name:
+                    req.future.set_exception(KeyError(f"store {name!r} was evicted"))
+                else:
+                    kept.append(req)
+            self._queue = kept
+            # wait for the passes in flight on this store to complete
+            while self._busy.get(name, 0):
+                self._cv.wait()
         res.native.release_device(self.device)
 
     @property
@@ -129,6 +142,9 @@ class Server:
             # future raises when its pass completes
             finite = torch.isfinite(x).all()
         x = x.reshape(res.order, -1)
+        if x.shape[1] > self.max_rhs:
+            raise ValueError(f"request of {x.shape[1]} right-hand sides exceeds the server's "
+                             f"max_rhs ({self.max_rhs}); split it")
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
         req = _Request(name, x, ready, int(x.shape[1]), time.perf_counter(), finite)
@@ -161,22 +177,31 @@ class Server:
         self.stop()
         return False
 
-    def _take(self) -> dict[str, list[_Request]]:
-        """Everything queued that fits the memory budget, grouped by store."""
+    def _take(self) -> dict:
+        """Everything queued that fits the memory budget, grouped by store
+        (name -> (resident store, requests)); called with the lock held."""
         groups: dict[str, list[_Request]] = collections.OrderedDict()
         rhs: dict[str, int] = {}
         kept = collections.deque()
         while self._queue:
             req = self._queue.popleft()
-            res = self._stores[req.store]
+            res = self._stores.get(req.store)
+            if res is None:  # evicted meanwhile
+                req.future.set_exception(KeyError(f"store {req.store!r} is not resident"))
+                continue
             k_now = rhs.get(req.store, 0)
             cost = 16 * res.order * req.k
             over_budget = (self._resident_bytes + self._inflight_bytes + cost > self.budget
                            and (groups or self._inflight_bytes))
-            if k_now + req.k > self.max_rhs or over_budget:
+            # a lone request always fits a pass (submit bounds its width)
+            if (k_now and k_now + req.k > self.max_rhs) or over_budget:
                 kept.append(req)  # waits for the next pass
                 continue
-            groups.setdefault(req.store, []).append(req)
+            if req.store not in groups:
+                # the store stays resident until this pass completes
+                self._busy[req.store] = self._busy.get(req.store, 0) + 1
+                groups[req.store] = (res, [])
+            groups[req.store][1].append(req)
             rhs[req.store] = k_now + req.k
             self._inflight_bytes += cost
         self._queue = kept
@@ -194,15 +219,16 @@ class Server:
                     return
                 groups = self._take()
             if not groups:
+                # everything queued waits for memory: until a pass completes
                 self._slots.release()
-                time.sleep(0)
+                with self._cv:
+                    self._cv.wait(timeout=0.001)
                 continue
             stream = self._streams[self._next_stream]
             self._next_stream = (self._next_stream + 1) % len(self._streams)
             work = []
             with torch.cuda.stream(stream):
-                for name, reqs in groups.items():
-                    res = self._stores[name]
+                for name, (res, reqs) in groups.items():
                     for r in reqs:
                         stream.wait_event(r.ready)
                     x = reqs[0].x if len(reqs) == 1 else torch.cat([r.x for r in reqs], dim=1)
@@ -242,7 +268,9 @@ class Server:
                 freed += cost
             with self._cv:
                 self._inflight_bytes -= freed
-                self._cv.notify()
+                for reqs, _, _ in work:
+                    self._busy[reqs[0].store] -= 1
+                self._cv.notify_all()
             self._slots.release()
 
     # -- statistics ---------------------------------------------------------------------------

@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the resident apply.
 
 The apply's one free launch parameter per call is how many right-hand sides a
-pass carries (1, 2, 4, 8 or 16; more columns take several passes).  Fewer per
-pass means fewer registers and more waves per CU but more passes over the
-codes; more per pass the reverse.  The result is bitwise the same for every
+pass carries: 1, 2 or 4 on the vector ALUs (the narrow passes) or 16, 32 or
+64 on the matrix cores (the wide passes); more columns take several passes.
+Fewer per pass means fewer registers and LDS and more waves per CU but more
+passes over the codes; more per pass the reverse.  The result is bitwise the same for every
 choice (each column's arithmetic is independent of the others), so the tuner
 only times: it applies the store with every candidate on device tensors,
 keeps the fastest per (store, k, precision) and records it in the store
@@ -18,7 +19,7 @@ from dataclasses import dataclass, field
 from . import _native as N
 from .device import precision_code, workspace_bytes
 
-CANDIDATES = (1, 2, 4, 8, 16)
+CANDIDATES = (1, 2, 4, 16, 32, 64)
 
 
 @dataclass

@@ -98,8 +98,11 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                          bit_widths: Sequence[int] = FULL_MENU, rank_tolerance: float = 0.1,
                          max_rank: int | None = None, truncation_fraction: float = 0.25,
                          arithmetic_reserve: float = 0.02, support_lower=None,
-                         support_upper=None, workers: int | None = None) -> HMatrix:
-    """Blockwise, parallel equivalent of from_dense(kernel(points, points), points, ...)."""
+                         support_upper=None, workers: int | None = None,
+                         progress: Callable[[int, int], None] | None = None) -> HMatrix:
+    """Blockwise, parallel equivalent of from_dense(kernel(points, points), points, ...).
+
+    ``progress(done, total)`` is called after every twentieth of the blocks."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     lo = p if support_lower is None else np.asarray(support_lower, dtype=np.float64)
     hi = p if support_upper is None else np.asarray(support_upper, dtype=np.float64)
@@ -131,8 +134,12 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
             pending.append((rect, pool.submit(work, rect)))
             if len(pending) >= window:
                 break
+        total, done, step = len(blocks), 0, max(1, len(blocks) // 20)
         while pending:
             rect, fut = pending.pop(0)
+            done += 1
+            if progress is not None and (done % step == 0 or done == total):
+                progress(done, total)
             kind, payload = fut.result()
             if kind == "dense":
                 native.add_dense(rect, payload)
@@ -158,10 +165,11 @@ class Workload:
     support_upper: np.ndarray | None = None
     note: str = ""
 
-    def build(self, workers: int | None = None) -> HMatrix:
+    def build(self, workers: int | None = None, progress=None) -> HMatrix:
         return build_kernel_hmatrix(self.points, self.kernel, tolerance=self.tolerance,
                                     leaf_size=self.leaf_size, support_lower=self.support_lower,
-                                    support_upper=self.support_upper, workers=workers)
+                                    support_upper=self.support_upper, workers=workers,
+                                    progress=progress)
 
     def dense(self) -> np.ndarray:
         return self.kernel(self.points, self.points)

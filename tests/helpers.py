@@ -90,10 +90,11 @@ def rel_err(a, b) -> float:
 
 
 # Right-hand sides per pass from which the device passes form the code
-# products on the matrix cores (narrower passes use the vector ALUs).  Within
-# one of the two paths a column's bits do not depend on the pass width; across
-# them results agree to the precision's tolerance (same_result).
-MATRIX_MIN_KB = 8
+# products on the matrix cores (the wide passes; 1, 2 or 4 per pass use the
+# vector ALUs).  Within one of the two paths a column's bits do not depend on
+# the pass width; across them results agree to the precision's tolerance
+# (same_result).
+MATRIX_MIN_KB = 16
 FP32_REL_TOL = 1e-6
 
 

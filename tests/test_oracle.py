@@ -267,3 +267,27 @@ def test_kernel_store_bitwise_equal_reference(tmp_path):
     s = O.OracleStore.from_file(mine)
     x = np.random.default_rng(1).normal(size=(300, 3))
     np.testing.assert_array_equal(s.matmul(x), r.matmul(x))
+
+
+def test_column_parallel_matmul_is_bitwise_matmul():
+    """matmul_columns (one right-hand side per thread) is the same function:
+    the reference loop nest never mixes columns."""
+    from tests import helpers as H
+
+    native = H.ragged_square(n=600, leaf=24).native
+    s = H.oracle_of(native)
+    x = np.random.default_rng(8).normal(size=(native.order, 7))
+    np.testing.assert_array_equal(s.matmul_columns(x, workers=4), s.matmul(x))
+
+
+@ref_only
+def test_reference_column_parallel_matmul_is_bitwise_matmul(tmp_path):
+    from tests import helpers as H
+
+    native = H.ragged_square(n=600, leaf=24).native
+    path = tmp_path / "store.qhmat"
+    native.save(path)
+    r = O.RefHMatrix.load(path)
+    x = np.random.default_rng(9).normal(size=(native.order, 6))
+    np.testing.assert_array_equal(r.matmul_columns(x, workers=3), r.matmul(x))
+    np.testing.assert_array_equal(r.matmul(x), O.OracleStore.from_file(path).matmul(x))

@@ -32,19 +32,23 @@ CASES = dict(_cases())
 
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("k", [1, 3])
-def test_plan_emulation_matches_oracle(name, k):
+@pytest.mark.parametrize("path", ["narrow", "wide_fp64", "wide_fp32split"])
+def test_plan_emulation_matches_oracle(name, k, path):
     native = CASES[name]
     orc = H.oracle_of(native)
     x = np.random.default_rng(k).normal(size=(native.order, k))
-    np.testing.assert_allclose(native.plan_emulate(x), orc.matmul(x), rtol=1e-12, atol=1e-12)
+    got = native.plan_emulate(x, wide=path != "narrow", precision=path.removeprefix("wide_"))
+    np.testing.assert_allclose(got, orc.matmul(x), rtol=1e-12, atol=1e-12)
 
 
 def test_rank_zero_and_dense_only_plans():
     source = np.diag([2.0, 3.0])
     rects = [qhmat.Rectangle(i, i + 1, j, j + 1) for i in range(2) for j in range(2)]
     h = qhmat.HMatrix.from_partition(source, [1, 0], rects, tolerance=0.1)
-    np.testing.assert_allclose(h.native.plan_emulate(np.ones(2)).ravel(), [2.0, 3.0])
+    for wide in (False, True):
+        np.testing.assert_allclose(h.native.plan_emulate(np.ones(2), wide=wide).ravel(), [2.0, 3.0])
     one = qhmat.NativeHMatrix(1, [0])
     one.add_dense(qhmat.Rectangle(0, 1, 0, 1), np.array([[4.0]]))
     one.finalize()
-    np.testing.assert_allclose(one.plan_emulate(np.array([0.5])).ravel(), [2.0])
+    for wide in (False, True):
+        np.testing.assert_allclose(one.plan_emulate(np.array([0.5]), wide=wide).ravel(), [2.0])
