@@ -267,25 +267,37 @@ def summary(leg: Leg, k: int, prec: str, y_ref) -> dict:
 def batch_stores(count: int, seed: int = 4, n_max: int = 16384):
     """Stores of uneven order, kernel, dimension and tolerance: n log-uniform
     in [1024, n_max], kernels 1/(r+0.4), exp(-r/0.2), log(r+1e-2) on 1-D, 2-D
-    and 3-D uniform points, tolerances 1e-4 and 1e-6 (1e-8 is infeasible for
-    the reference constructor at these sizes), right-hand-side counts uniform
-    in 1..16."""
+    and 3-D uniform points, tolerances drawn from {1e-4, 1e-6, 1e-8},
+    right-hand-side counts uniform in 1..16.  A store the reference
+    constructor refuses (its width budget infeasible: every 1e-8 store, and
+    the larger 1e-6 ones) is built as a labelled stress store: ranks from the
+    SVD tail, every component 32 bits (workloads.build_kernel_hmatrix)."""
     import numpy as np
 
     from qhmat_amd import workloads as W
+    from qhmat_amd.matrix import InfeasibleError
 
     rng = np.random.default_rng(seed)
     kernels = [W.inverse_shifted, W.exponential, W.log_kernel]
     natives, ks, shapes = [], [], []
     for i in range(count):
         n = int(round(np.exp(rng.uniform(np.log(1024), np.log(n_max)))))
-        dim = 1 + i % 3
-        tol = (1e-4, 1e-6)[i % 2]
+        dim = int(rng.integers(1, 4))
+        kern = kernels[int(rng.integers(0, 3))]
+        tol = (1e-4, 1e-6, 1e-8)[int(rng.integers(0, 3))]
         pts = W.uniform_points(n, dim, seed=100 + i)
-        h = W.build_kernel_hmatrix(pts, kernels[i % 3], tolerance=tol, leaf_size=64)
+        stress = tol <= 1e-8
+        if not stress:
+            try:
+                h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64)
+            except InfeasibleError:
+                stress = True
+        if stress:
+            h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, stress_width=32)
         natives.append(h.native)
         ks.append(int(rng.integers(1, 17)))
-        shapes.append((n, dim, kernels[i % 3].__name__, tol))
+        shapes.append({"n": n, "dim": dim, "kernel": kern.__name__, "tol": tol,
+                       "stress": stress})
     return natives, ks, shapes
 
 

@@ -280,7 +280,8 @@ class RefHMatrix:
             raise RuntimeError(r.ref_last_error().decode())
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # module globals may already be gone at interpreter shutdown
+        if getattr(self, "_h", None) and callable(ref):
             ref().ref_destroy(self._h)
             self._h = None
 

@@ -1282,16 +1282,18 @@ void build_wide(Plan& plan, const std::vector<ItemDev>& right_items,
         const std::uint64_t cap = std::max<std::uint64_t>(1, std::min<std::uint64_t>(8, uitems[t].size()));
         const std::uint32_t runs =
             std::uint32_t(std::clamp<std::uint64_t>((cost + lunit / 2) / lunit, 1, cap));
-        std::vector<std::uint32_t> cut(runs + 1);
-        split_runs(lc[t], runs, std::uint32_t(out.size()), cut.data());
+        // each run is a workgroup; its items are cut again into one share
+        // per wave
+        std::vector<std::uint32_t> cut(runs * kWideWaves + 1);
+        split_runs(lc[t], runs * kWideWaves, std::uint32_t(out.size()), cut.data());
         out.insert(out.end(), uitems[t].begin(), uitems[t].end());
-        if (runs == 1) {
-            plan.wruns_l.push_back(WideRun{std::uint32_t(t), cut[0], cut[1], kNoSet});
-            continue;
-        }
-        const std::uint32_t set0 = plan.wsets;
+        const std::uint32_t set0 = runs == 1 ? kNoSet : plan.wsets;
         for (std::uint32_t s = 0; s < runs; ++s)
-            plan.wruns_l.push_back(WideRun{std::uint32_t(t), cut[s], cut[s + 1], set0 + s});
+            for (int w = 0; w < kWideWaves; ++w)
+                plan.wruns_l.push_back(WideRun{std::uint32_t(t), cut[s * kWideWaves + w],
+                                               cut[s * kWideWaves + w + 1],
+                                               runs == 1 ? kNoSet : set0 + s});
+        if (runs == 1) continue;
         plan.wsums.push_back(WideSum{std::uint32_t(t), set0, runs, 0});
         plan.wsets += runs;
     }
@@ -1746,7 +1748,7 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
     s->n_items = std::uint32_t(plan.items.size());
     s->z_rows = plan.z_rows;
     s->n_wr_groups = std::uint32_t(plan.wruns_r.size() / kWideWaves);
-    s->n_wl_groups = std::uint32_t(plan.wruns_l.size());
+    s->n_wl_groups = std::uint32_t(plan.wruns_l.size() / kWideWaves);
     s->n_wsums = std::uint32_t(plan.wsums.size());
     s->n_wzsums = std::uint32_t(plan.wzsums.size());
     s->wsets = plan.wsets;
@@ -2044,17 +2046,22 @@ void emulate_wide(const Plan& pl, const double* x, std::int64_t k, double* y, bo
                 put(std::uint64_t(zr.vzrow) + v, p, s);
             }
     }
-    // pass L
+    // pass L: workgroups of kWideWaves wave runs on one row tile and set
+    check(pl.wruns_l.size() % kWideWaves == 0, "pass L runs are not whole workgroups");
     std::vector<double> sets(std::uint64_t(pl.wsets) * kTile * K, 0.0);
     std::vector<int> tile_runs(pl.tiles.size(), 0), direct(pl.tiles.size(), 0);
-    for (const WideRun& run : pl.wruns_l) {
-        const TileDev& T = at(pl.tiles, run.tile);
-        const std::uint32_t R = T.rows;
-        check(R >= 1 && R <= std::uint32_t(kTile), "tile rows");
-        check(run.begin <= run.end && run.end <= pl.witems.size(), "pass L run range");
-        ++tile_runs[run.tile];
-        std::vector<double> acc(std::uint64_t(kTile) * K, 0.0);
-        for (std::uint32_t i = run.begin; i < run.end; ++i) {
+    for (std::size_t g = 0; g < pl.wruns_l.size() / kWideWaves; ++g) {
+      const WideRun& run = pl.wruns_l[g * kWideWaves];
+      const TileDev& T = at(pl.tiles, run.tile);
+      const std::uint32_t R = T.rows;
+      check(R >= 1 && R <= std::uint32_t(kTile), "tile rows");
+      ++tile_runs[run.tile];
+      std::vector<double> acc(std::uint64_t(kTile) * K, 0.0);
+      for (int wv = 0; wv < kWideWaves; ++wv) {
+        const WideRun& wr = pl.wruns_l[g * kWideWaves + wv];
+        check(wr.tile == run.tile && wr.set == run.set, "a pass-L workgroup spans two runs");
+        check(wr.begin <= wr.end && wr.end <= pl.witems.size(), "pass L run range");
+        for (std::uint32_t i = wr.begin; i < wr.end; ++i) {
             const ItemDev& it = pl.witems[i];
             ++seen[i];
             if (it.kind == kDense) {
@@ -2098,6 +2105,7 @@ void emulate_wide(const Plan& pl, const double* x, std::int64_t k, double* y, bo
                 }
             }
         }
+      }
         for (std::uint32_t r = 0; r < R; ++r)
             for (std::uint64_t p = 0; p < K; ++p) {
                 if (run.set == kNoSet) {

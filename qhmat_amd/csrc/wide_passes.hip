@@ -111,6 +111,8 @@ __device__ __forceinline__ int code_part(int code, std::uint32_t part) {
 // in a staged right side block (word w of column c at side[64 w + (c xor
 // 4 (w mod 16))]): two aligned four-word reads per four columns.  The head
 // row below the block and the slack row past it are readable.
+// SPLIT: the rows may carry split parts; otherwise every row is whole.
+template <bool SPLIT>
 __device__ __forceinline__ void decode_right16(const std::uint32_t* side, std::uint32_t m, int q,
                                                int (&code)[16]) {
     const std::uint32_t E = vrow_end(m);
@@ -123,9 +125,10 @@ __device__ __forceinline__ void decode_right16(const std::uint32_t* side, std::u
         const quad_t lo =
             *reinterpret_cast<const quad_t*>(side + 64 * (kk - 1) + (c4 ^ (4 * ((kk - 1) & 15))));
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            code[4 * u + i] =
-                code_part(static_cast<int>(__builtin_amdgcn_alignbit(hi[i], lo[i], E)) >> sh, part);
+        for (int i = 0; i < 4; ++i) {
+            const int c = static_cast<int>(__builtin_amdgcn_alignbit(hi[i], lo[i], E)) >> sh;
+            code[4 * u + i] = SPLIT ? code_part(c, part) : c;
+        }
     }
 }
 
@@ -259,21 +262,35 @@ __device__ __forceinline__ void right_group(const WideView& V, std::uint32_t g, 
         const std::uint32_t nv = build_vrows<FP32>(bits, cur.count, lane, vrow);
         wave_lds_sync();
 
+        // rows 2 W .. nv - 1 are whole components: their groups skip the
+        // split handling
+        const std::uint32_t split_rows = 2u * (nv - cur.count);
         for (std::uint32_t g0 = 0; g0 < nv; g0 += 16) {
             const std::uint32_t va = g0 + std::uint32_t(a_row<FP32>(r));
             const std::uint32_t m = vrow_at(vrow, va, nv);
             int code[16];
-            decode_right16(stage, m, q, code);
+            if (g0 < split_rows)
+                decode_right16<true>(stage, m, q, code);
+            else
+                decode_right16<false>(stage, m, q, code);
             Real a[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) a[j] = Real(code[j]);
-            // the output rows of this lane: virtual rows g0 + 4 q + t
-            std::uint32_t om[4];
+            // this lane's output rows (virtual rows g0 + 4 q + t): part, the
+            // component's scale and the element offset of right-hand side 0
+            std::uint32_t part[4];
+            double sc[4];
+            std::uint64_t at[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const std::uint32_t v = g0 + 4u * std::uint32_t(q) + std::uint32_t(t);
                 const std::uint32_t mv = vrow[min(v, 127u)];
-                om[t] = v < nv ? mv : (kLow << 6);  // past the rows: nothing to write
+                part[t] = v < nv ? vrow_part(mv) : kLow;  // past the rows: nothing to write
+                const std::uint32_t j = vrow_comp(mv);
+                sc[t] = scl[j];
+                // per component, or in virtual-row order (a split
+                // component's two rows both carry its sum)
+                at[t] = (std::uint64_t(cur.zrow) + (cur.nseg ? v : j)) * std::uint64_t(k) + p0 + r;
             }
 #pragma unroll 2
             for (int n = 0; n < S::NS; ++n) {
@@ -281,24 +298,17 @@ __device__ __forceinline__ void right_group(const WideView& V, std::uint32_t g, 
                 typename Mm::acc d = Mm::zero();
 #pragma unroll
                 for (int j = 0; j < 16; ++j) d = Mm::mad(a[j], xr[j], d);
-                const int p = p0 + 16 * n + r;
-                if (p < k) {
+                if (p0 + 16 * n + r < k) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
-                        const std::uint32_t part = vrow_part(om[t]);
-                        if (part == kLow) continue;
+                        if (part[t] == kLow) continue;
                         // a high row sits at an even t, its low row at t + 1
                         double z = double(d[t]);
-                        if (part == kHigh) z = z * 65536.0 + double(d[(t + 1) & 3]);
-                        const std::uint32_t j = vrow_comp(om[t]);
-                        const Real zs = Real(scl[j] * z);
-                        // per component, or in virtual-row order (a split
-                        // component's two rows both carry zs)
-                        const std::uint32_t v = g0 + 4u * std::uint32_t(q) + std::uint32_t(t);
-                        const std::uint64_t row = std::uint64_t(cur.zrow) + (cur.nseg ? v : j);
-                        zout[row * std::uint64_t(k) + std::uint64_t(p)] = zs;
-                        if (cur.nseg && part == kHigh)
-                            zout[(row + 1) * std::uint64_t(k) + std::uint64_t(p)] = zs;
+                        if (part[t] == kHigh) z = z * 65536.0 + double(d[(t + 1) & 3]);
+                        const Real zs = Real(sc[t] * z);
+                        Real* o = zout + at[t] + 16 * n;
+                        o[0] = zs;
+                        if (cur.nseg && part[t] == kHigh) o[k] = zs;
                     }
                 }
             }
@@ -308,13 +318,27 @@ __device__ __forceinline__ void right_group(const WideView& V, std::uint32_t g, 
 }
 
 
-// What a pass-L wave fetches of an item one item ahead: its 16 rows of the
-// left side block (one 16-byte group per lane: word row l / 4, words
-// 4 (l mod 4) .. + 3 of the 16), the component widths (lanes = components)
-// and the B operands of the first four K-steps.  A diagonal or empty item
-// fetches nothing (every load out of range).
+// Pass L.  A workgroup owns one run of a row tile's items; its kWideWaves
+// waves each take a share of the run (their own wave runs) and form all 64
+// rows of the tile from it: per K-step a lane decodes the codes of four rows
+// (one per 16-row block) and the B operands are shared by the four blocks.
+// The waves' FP64 sums are added in a fixed order at the end.
+template <int KC, bool FP32> struct LeftShape {
+    static constexpr int NS = KC / 16;
+    static constexpr int kVrow = kLdsHead + kStageBytes;  // per wave: head, stage, rows
+    static constexpr int kWaveBytes = kVrow + 128 * 4;
+    // the final sums, one 16-wide slice at a time: [wave][row block][t][lane]
+    static constexpr int kSumBytes = kWideWaves * 4 * 4 * 64 * 8;
+    static constexpr int kBytes = kWideWaves * kWaveBytes > kSumBytes ? kWideWaves * kWaveBytes
+                                                                       : kSumBytes;
+};
+
+// What a pass-L wave fetches of an item one item ahead: the item's left side
+// block (four 16-byte groups per lane), the component widths (lanes =
+// components) and the B operands of its first four K-steps.  A diagonal or
+// empty item fetches nothing (every load out of range).
 template <int KC, bool FP32> struct LeftFetch {
-    quad_t q;
+    quad_t q[kItemQuads / 64];
     std::uint32_t bits;
     real_t<FP32> b[4][KC / 16];
 };
@@ -351,93 +375,116 @@ __device__ __forceinline__ void left_b(const WideView& V, const ItemDev& f, std:
 }
 
 template <int KC, bool FP32>
-__device__ __forceinline__ void left_fetch(const WideView& V, const ItemDev& f, std::uint32_t rb,
-                                           int lane, int p0, LeftFetch<KC, FP32>& F) {
+__device__ __forceinline__ void left_fetch(const WideView& V, const ItemDev& f, int lane, int p0,
+                                           LeftFetch<KC, FP32>& F) {
     const bool u = f.kind == kLeft && f.count > 0;
-    // word row w of rows rb .. rb + 15 at 64 w + rb: quads 16 w + rb / 4 + (0..3);
-    // rows past the block's lie past the buffer and read zero
     const __amdgpu_buffer_rsrc_t data =
         buffer(V.words + 4ull * f.quad, u ? std::uint32_t(f.quads) * 16u : 0u);
-    F.q = __builtin_amdgcn_raw_buffer_load_b128(
-        data, (16u * (std::uint32_t(lane) >> 2) + rb / 4u + (std::uint32_t(lane) & 3u)) * 16u, 0, 0);
+#pragma unroll
+    for (int t = 0; t < kItemQuads / 64; ++t)
+        F.q[t] = __builtin_amdgcn_raw_buffer_load_b128(data, std::uint32_t(lane + 64 * t) * 16u, 0, 0);
     F.bits = __builtin_amdgcn_raw_buffer_load_b32(buffer(V.widths + f.comp0, u ? f.count * 4u : 0u),
                                                   std::uint32_t(lane) * 4u, 0, 0);
     left_b<KC, FP32>(V, f, 0, lane >> 4, lane & 15, p0, F.b);
 }
 
-// One fetched item of a pass-L run, added to this wave's rows.
+// One fetched item of a pass-L wave run, added to the wave's sums of all 64
+// rows (sums[rb][n] slot t: row 16 rb + 4 q + t, right-hand side p0 + 16 n + r).
 template <int KC, bool FP32>
 __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
-                                          const LeftFetch<KC, FP32>& F, std::uint32_t rb,
-                                          std::uint32_t R, int lane, int p0, std::uint32_t* stage,
-                                          std::uint32_t* vrow, f64x4 (&sums)[KC / 16]) {
+                                          const LeftFetch<KC, FP32>& F, std::uint32_t R, int lane,
+                                          int p0, std::uint8_t* wl, f64x4 (&sums)[4][KC / 16]) {
     using Real = real_t<FP32>;
+    using S = LeftShape<KC, FP32>;
     constexpr int NS = KC / 16;
     const int q = lane >> 4, r = lane & 15;
     const int k = V.k;
     if (f.count == 0) return;  // past the end of the run
     if (f.kind == kDense) {
         // R x cnt FP64 values, column-major; lane (q, r) supplies row
-        // rb + a_row(r) of column 4 s + q and x of that column (buffer loads
-        // past the item's values and columns read zero; x at a valid
-        // address, the columns past k never written)
+        // 16 rb + a_row(r) of column 4 s + q and x of that column (buffer
+        // loads past the item's values and columns read zero; rows past the
+        // tile read its last row and are never written; x at a valid address,
+        // the columns past k never written)
         const std::uint32_t cnt = min(std::uint32_t(f.count), std::uint32_t(kDenseColumns));
         const __amdgpu_buffer_rsrc_t dv = buffer(V.words + 4ull * f.quad, cnt * R * 8u);
         const __amdgpu_buffer_rsrc_t pc = buffer(V.perm + f.col, cnt * 4u);
-        const std::uint32_t row = rb + std::uint32_t(a_row<false>(r));
-        double a[kDenseColumns / 4], b[kDenseColumns / 4][NS];
+        double b[kDenseColumns / 4][NS];
 #pragma unroll
         for (int s = 0; s < kDenseColumns / 4; ++s) {
             const std::uint32_t c = 4u * std::uint32_t(s) + std::uint32_t(q);
-            // rows past the tile read its last row (their results are never written)
-            a[s] = load_f64(dv, (c * R + min(row, R - 1u)) * 8u);
             const std::int64_t xrow =
                 std::int64_t(__builtin_amdgcn_raw_buffer_load_b32(pc, c * 4u, 0, 0)) * V.ldx;
 #pragma unroll
             for (int n = 0; n < NS; ++n) b[s][n] = V.x[xrow + min(p0 + 16 * n + r, k - 1)];
         }
 #pragma unroll
-        for (int s = 0; s < kDenseColumns / 4; ++s)
+        for (int rb = 0; rb < 4; ++rb) {
+            const std::uint32_t row = min(16u * rb + std::uint32_t(a_row<false>(r)), R - 1u);
 #pragma unroll
-            for (int n = 0; n < NS; ++n) sums[n] = Mma<false>::mad(a[s], b[s][n], sums[n]);
+            for (int s = 0; s < kDenseColumns / 4; ++s) {
+                const std::uint32_t c = 4u * std::uint32_t(s) + std::uint32_t(q);
+                const double a = load_f64(dv, (c * R + row) * 8u);
+#pragma unroll
+                for (int n = 0; n < NS; ++n) sums[rb][n] = Mma<false>::mad(a, b[s][n], sums[rb][n]);
+            }
+        }
         return;
     }
-    reinterpret_cast<quad_t*>(stage)[lane] = F.q;
+    quad_t* stage_q = reinterpret_cast<quad_t*>(wl + kLdsHead);
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wl + kLdsHead);
+    std::uint32_t* vrow = reinterpret_cast<std::uint32_t*>(wl + S::kVrow);
+#pragma unroll
+    for (int t = 0; t < kItemQuads / 64; ++t)
+        if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
     wave_lds_sync();
     const std::uint32_t nv = build_vrows<FP32>(F.bits, f.count, lane, vrow);
     wave_lds_sync();
     const std::uint32_t arow = FP32 ? std::uint32_t(r) : std::uint32_t(a_row<false>(r));
-    typename Mma<FP32>::acc acc[NS];
+    typename Mma<FP32>::acc acc[4][NS];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) acc[n] = Mma<FP32>::zero();
-    // A: the code of row rb + arow (lanes past the tile's rows read the zero
-    // words of the block's padding lanes); a high part carries its 2^16
-    // (exact), rows past the item are zero
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int n = 0; n < NS; ++n) acc[rb][n] = Mma<FP32>::zero();
+    // A: the codes of rows 16 rb + arow (lanes past the tile's rows read the
+    // zero words of the block's padding lanes); a high part carries its 2^16
+    // (exact); rows past the item are zero
+    // K-steps from row 2 W on are whole components (no split handling), and
+    // only the last one can reach past the rows
+    const std::uint32_t split_rows = 2u * (nv - std::uint32_t(f.count));
     auto run_chunk = [&](std::uint32_t c, const Real (&b)[4][NS]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const std::uint32_t v = 16u * c + 4u * std::uint32_t(s) + std::uint32_t(q);
+            const std::uint32_t v0 = 16u * c + 4u * std::uint32_t(s);
+            const std::uint32_t v = v0 + std::uint32_t(q);
             const std::uint32_t m = vrow_at(vrow, v, nv);
             const std::uint32_t E = vrow_end(m);
-            const std::uint32_t kk = E >> 5;
-            const std::uint32_t hi = stage[16 * kk + arow];
-            const std::uint32_t lo = stage[16 * (int(kk) - 1) + int(arow)];
-            const int code = code_part(
-                static_cast<int>(__builtin_amdgcn_alignbit(hi, lo, E)) >> (32u - vrow_bits(m)),
-                vrow_part(m));
-            const Real av = vrow_part(m) == kHigh ? Real(code) * Real(65536) : Real(code);
-            const Real a = v < nv ? av : Real(0);
+            const std::uint32_t kk = E >> 5, sh = 32u - vrow_bits(m), part = vrow_part(m);
+            const std::uint32_t* hi = stage + 64 * kk + arow;
+            const std::uint32_t* lo = stage + 64 * (int(kk) - 1) + int(arow);
+            const bool split = v0 < split_rows, edge = v0 + 4u > nv;
 #pragma unroll
-            for (int n = 0; n < NS; ++n) acc[n] = Mma<FP32>::mad(a, b[s][n], acc[n]);
+            for (int rb = 0; rb < 4; ++rb) {
+                int code = static_cast<int>(__builtin_amdgcn_alignbit(hi[16 * rb], lo[16 * rb], E)) >> sh;
+                Real a;
+                if (split) {
+                    code = code_part(code, part);
+                    a = part == kHigh ? Real(code) * Real(65536) : Real(code);
+                } else {
+                    a = Real(code);
+                }
+                if (edge) a = v < nv ? a : Real(0);
+#pragma unroll
+                for (int n = 0; n < NS; ++n) acc[rb][n] = Mma<FP32>::mad(a, b[s][n], acc[rb][n]);
+            }
         }
     };
     // chunk 0 came with the fetch; later ones load one chunk ahead (both
     // halves of the loop always run: an odd count runs one chunk past the
     // rows, whose A operands are zero)
     const std::uint32_t nchunks = (nv + 15u) / 16u;
-    Real b1[4][NS];
     if (nchunks > 1) {
-        Real b0[4][NS];
+        Real b0[4][NS], b1[4][NS];
         left_b<KC, FP32>(V, f, 1, q, r, p0, b1);
         run_chunk(0, F.b);
         for (std::uint32_t c = 1; c < nchunks; c += 2) {
@@ -450,20 +497,13 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
         run_chunk(0, F.b);
     }
 #pragma unroll
-    for (int n = 0; n < NS; ++n)
+    for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) sums[n][t] += double(acc[n][t]);
+        for (int n = 0; n < NS; ++n)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sums[rb][n][t] += double(acc[rb][n][t]);
     wave_lds_sync();  // the next item overwrites the stage and the rows
 }
-
-// Pass L, workgroup g: wave w forms rows 16 w .. 16 w + 15 of the run's row
-// tile for right-hand sides [p0, p0 + KC), one item fetched ahead.
-template <int KC, bool FP32> struct LeftShape {
-    static constexpr int NS = KC / 16;
-    static constexpr int kStageWords = 18 * 16;  // head row, 16 word rows, slack row (16 lanes)
-    static constexpr int kWaveBytes = kStageWords * 4 + 128 * 4;
-    static constexpr int kBytes = kWideWaves * kWaveBytes;
-};
 
 template <int KC, bool FP32>
 __device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, int p0,
@@ -473,53 +513,62 @@ __device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, i
     const int tid = int(threadIdx.x);
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63, q = lane >> 4, r = lane & 15;
-    const WideRun run = V.lruns[g];
+    const WideRun run = V.lruns[g * kWideWaves + wave];
     const std::uint32_t begin = __builtin_amdgcn_readfirstlane(run.begin);
     const std::uint32_t end = __builtin_amdgcn_readfirstlane(run.end);
     const TileDev* __restrict__ T = V.tiles + run.tile;
     const std::uint32_t R = min(T->rows, std::uint32_t(kTile));
     const std::uint32_t row0 = T->row0;
     const int k = V.k;
-    const std::uint32_t rb = 16u * std::uint32_t(wave);  // first row of this wave
-
     std::uint8_t* wl = lds + wave * S::kWaveBytes;
-    std::uint32_t* stage = reinterpret_cast<std::uint32_t*>(wl) + 16;  // word row 0
-    std::uint32_t* vrow = reinterpret_cast<std::uint32_t*>(wl + S::kStageWords * 4);
 
-    f64x4 sums[NS];
+    f64x4 sums[4][NS];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) sums[n] = Mma<false>::zero();
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int n = 0; n < NS; ++n) sums[rb][n] = Mma<false>::zero();
 
     // Two fetch buffers alternate, so no register of a load in flight is
     // copied; both halves of the loop always run (an odd count ends with
     // an empty item).
     LeftFetch<KC, FP32> FA, FB;
     ItemDev fa = item_or_empty(V, begin, end);
-    left_fetch<KC, FP32>(V, fa, rb, lane, p0, FA);
+    left_fetch<KC, FP32>(V, fa, lane, p0, FA);
     for (std::uint32_t i = begin; i < end; i += 2) {
         const ItemDev fb = item_or_empty(V, i + 1, end);
-        left_fetch<KC, FP32>(V, fb, rb, lane, p0, FB);
-        left_item<KC, FP32>(V, fa, FA, rb, R, lane, p0, stage, vrow, sums);
+        left_fetch<KC, FP32>(V, fb, lane, p0, FB);
+        left_item<KC, FP32>(V, fa, FA, R, lane, p0, wl, sums);
         fa = item_or_empty(V, i + 2, end);
-        left_fetch<KC, FP32>(V, fa, rb, lane, p0, FA);
-        left_item<KC, FP32>(V, fb, FB, rb, R, lane, p0, stage, vrow, sums);
+        left_fetch<KC, FP32>(V, fa, lane, p0, FA);
+        left_item<KC, FP32>(V, fb, FB, R, lane, p0, wl, sums);
     }
 
-    // rows rb + 4 q + t, right-hand side p0 + 16 n + r
+    // The four waves' sums, one slice at a time, through LDS: wave w adds
+    // row block w of every wave as (s0 + s1) + (s2 + s3) and writes rows
+    // 16 w + 4 q + t of right-hand side p0 + 16 n + r.
+    double* part = reinterpret_cast<double*>(lds);
     double* sets = reinterpret_cast<double*>(static_cast<std::uint8_t*>(V.zw) +
                                              V.z_rows * std::uint64_t(k) * sizeof(double));
-#pragma unroll
     for (int n = 0; n < NS; ++n) {
+        __syncthreads();  // the stage (the first slice) or the previous slice is done with
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) part[((wave * 4 + rb) * 4 + t) * 64 + lane] = sums[rb][n][t];
+        __syncthreads();
         const int p = p0 + 16 * n + r;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const std::uint32_t row = rb + 4u * std::uint32_t(q) + std::uint32_t(t);
+            double v[kWideWaves];
+#pragma unroll
+            for (int w = 0; w < kWideWaves; ++w) v[w] = part[((w * 4 + wave) * 4 + t) * 64 + lane];
+            const double sum = (v[0] + v[1]) + (v[2] + v[3]);
+            const std::uint32_t row = 16u * std::uint32_t(wave) + 4u * std::uint32_t(q) + std::uint32_t(t);
             if (row < R && p < k) {
                 if (run.set == kNoSet)
-                    V.y[std::int64_t(V.perm[row0 + row]) * V.ldy + p] = sums[n][t];
+                    V.y[std::int64_t(V.perm[row0 + row]) * V.ldy + p] = sum;
                 else
-                    sets[(std::uint64_t(run.set) * kTile + row) * std::uint64_t(k) + std::uint64_t(p)] =
-                        sums[n][t];
+                    sets[(std::uint64_t(run.set) * kTile + row) * std::uint64_t(k) + std::uint64_t(p)] = sum;
             }
         }
     }
@@ -577,9 +626,10 @@ __device__ __forceinline__ void sum_group(const WideView& V, std::uint32_t g) {
 // Waves per SIMD the register allocation must allow: pass R; pass L, whose
 // two fetch buffers grow with the right-hand sides per pass.
 template <bool FP32> constexpr int wide_occupancy() { return FP32 ? 4 : 3; }
-template <int KC, bool FP32> constexpr int left_occupancy() {
-    return KC <= 16 ? 4 : (KC <= 32 ? (FP32 ? 4 : 3) : (FP32 ? 3 : 2));
-}
+// Pass L runs 16 right-hand sides per launch slice whatever pass R's width:
+// its waves hold FP64 sums of all 64 rows, 32 registers per 16 columns.
+constexpr int kLeftKC = 16;
+template <bool FP32> constexpr int left_occupancy() { return FP32 ? 4 : 3; }
 
 template <int KC, bool FP32>
 __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_right(WideView V) {
@@ -587,10 +637,10 @@ __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_
     right_group<KC, FP32>(V, blockIdx.x, int(blockIdx.y) * KC, lds);
 }
 
-template <int KC, bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, (left_occupancy<KC, FP32>())) void wide_left(WideView V) {
-    __shared__ __align__(16) std::uint8_t lds[LeftShape<KC, FP32>::kBytes];
-    left_group<KC, FP32>(V, blockIdx.x, int(blockIdx.y) * KC, lds);
+template <bool FP32>
+__global__ __launch_bounds__(64 * kWideWaves, left_occupancy<FP32>()) void wide_left(WideView V) {
+    __shared__ __align__(16) std::uint8_t lds[LeftShape<kLeftKC, FP32>::kBytes];
+    left_group<kLeftKC, FP32>(V, blockIdx.x, int(blockIdx.y) * kLeftKC, lds);
 }
 
 __global__ __launch_bounds__(256) void wide_sum(WideView V) { sum_group(V, blockIdx.x); }
@@ -633,16 +683,16 @@ __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void batch
     right_group<KC, FP32>(V, blockIdx.x - first[m], p0, lds);
 }
 
-template <int KC, bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, (left_occupancy<KC, FP32>())) void batch_wide_left(
+template <bool FP32>
+__global__ __launch_bounds__(64 * kWideWaves, 3) void batch_wide_left(
     const WideView* __restrict__ members, const std::uint32_t* __restrict__ first,
     std::uint32_t count, const double* x, double* y, std::uint8_t* zw) {
-    __shared__ __align__(16) std::uint8_t lds[LeftShape<KC, FP32>::kBytes];
+    __shared__ __align__(16) std::uint8_t lds[LeftShape<kLeftKC, FP32>::kBytes];
     const std::uint32_t m = owner(first, count, blockIdx.x);
     const WideView V = bind(members, m, x, y, zw);
-    const int p0 = int(blockIdx.y) * KC;
+    const int p0 = int(blockIdx.y) * kLeftKC;
     if (p0 >= V.k) return;
-    left_group<KC, FP32>(V, blockIdx.x - first[m], p0, lds);
+    left_group<kLeftKC, FP32>(V, blockIdx.x - first[m], p0, lds);
 }
 
 template <bool FP32>
@@ -681,8 +731,8 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
         hip_check(hipGetLastError(), "wide partial-sum reduction launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
-    hipLaunchKernelGGL((wide_left<KC, FP32>), dim3(c.left_groups, passes), dim3(64 * kWideWaves),
-                       0, st, V);
+    hipLaunchKernelGGL((wide_left<FP32>), dim3(c.left_groups, unsigned((V.k + kLeftKC - 1) / kLeftKC)),
+                       dim3(64 * kWideWaves), 0, st, V);
     hip_check(hipGetLastError(), "wide pass L launch");
     if (c.sums) {
         hipLaunchKernelGGL(wide_sum, dim3(c.sums), dim3(256), 0, st, V);
@@ -706,7 +756,8 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
                            c.members, c.first_zsum, n, x, y, zw, KC);
         hip_check(hipGetLastError(), "batched wide partial-sum reduction launch");
     }
-    hipLaunchKernelGGL((batch_wide_left<KC, FP32>), dim3(c.left_groups, passes),
+    hipLaunchKernelGGL((batch_wide_left<FP32>),
+                       dim3(c.left_groups, unsigned((c.max_k + kLeftKC - 1) / kLeftKC)),
                        dim3(64 * kWideWaves), 0, st, c.members, c.first_left, n, x, y, zw);
     hip_check(hipGetLastError(), "batched wide pass L launch");
     if (c.sums) {

@@ -99,10 +99,19 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                          max_rank: int | None = None, truncation_fraction: float = 0.25,
                          arithmetic_reserve: float = 0.02, support_lower=None,
                          support_upper=None, workers: int | None = None,
-                         progress: Callable[[int, int], None] | None = None) -> HMatrix:
+                         progress: Callable[[int, int], None] | None = None,
+                         stress_width: int | None = None) -> HMatrix:
     """Blockwise, parallel equivalent of from_dense(kernel(points, points), points, ...).
 
-    ``progress(done, total)`` is called after every twentieth of the blocks."""
+    ``progress(done, total)`` is called after every twentieth of the blocks.
+
+    ``stress_width``: build a store the reference constructor refuses (a
+    tolerance whose width budget no allocation meets, such as 1e-8 on the
+    2-D and 3-D kernels): each block keeps the rank its truncation allowance
+    needs (the SVD tail, as matrix.py:213-238 chooses it) and every component
+    gets this width, added through add_low_rank (core.cpp:284-330), which
+    checks no budget.  The store is labelled by its use: a stress case of the
+    widest codes, not a reference-constructible store."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     lo = p if support_lower is None else np.asarray(support_lower, dtype=np.float64)
     hi = p if support_upper is None else np.asarray(support_upper, dtype=np.float64)
@@ -117,6 +126,9 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                                             clustered[rect.col0:rect.col1]))
         if rect.row0 == rect.col0 and rect.row1 == rect.col1:
             return ("dense", block)
+        if stress_width is not None:
+            return ("lowrank", _tail_rank_factors(block, tolerance, rank_tolerance,
+                                                  truncation_fraction, stress_width))
         return ("lowrank", factor_block(block, rect, tolerance=tolerance, widths=widths,
                                         rank_tolerance=rank_tolerance, max_rank=max_rank,
                                         truncation_fraction=truncation_fraction,
@@ -153,6 +165,24 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
     return HMatrix(native, source_array_bytes=n * n * 8)
 
 
+def _tail_rank_factors(block, tolerance, rank_tolerance, truncation_fraction, width):
+    """(u, sigma, v, bits, residual) at the rank the truncation allowance
+    needs, every component `width` bits (no allocation budget)."""
+    budget = tolerance * float(np.max(np.abs(block)))
+    m, n = block.shape
+    if budget == 0.0:
+        return np.zeros((m, 0)), np.empty(0), np.zeros((n, 0)), [], 0.0
+    u, sigma, vt = np.linalg.svd(block, full_matrices=False)
+    tail = np.sqrt(np.concatenate([np.cumsum(sigma[::-1] ** 2)[::-1], [0.0]]))
+    allowance = min(rank_tolerance * tail[0], truncation_fraction * budget)
+    fits = np.nonzero(tail[1:] <= allowance)[0]
+    rank = int(fits[0]) + 1 if fits.size else sigma.size
+    rank = min(rank, int(np.count_nonzero(sigma > 0)))
+    u, sigma, v = u[:, :rank], sigma[:rank], vt[:rank, :].T
+    residual = float(np.max(np.abs(block - (u * sigma) @ v.T)))
+    return u, sigma, v, [int(width)] * rank, residual
+
+
 @dataclass
 class Workload:
     name: str
@@ -165,11 +195,18 @@ class Workload:
     support_upper: np.ndarray | None = None
     note: str = ""
 
-    def build(self, workers: int | None = None, progress=None) -> HMatrix:
+    def build(self, workers: int | None = None, progress=None,
+              stress_width: int | None = None) -> HMatrix:
         return build_kernel_hmatrix(self.points, self.kernel, tolerance=self.tolerance,
                                     leaf_size=self.leaf_size, support_lower=self.support_lower,
                                     support_upper=self.support_upper, workers=workers,
-                                    progress=progress)
+                                    progress=progress, stress_width=stress_width)
+
+    def rows(self, index) -> np.ndarray:
+        """Rows `index` (source order) of the dense kernel matrix, evaluated
+        from the points: the exact A x of selected rows without the n x n
+        matrix."""
+        return self.kernel(self.points[np.asarray(index)], self.points)
 
     def dense(self) -> np.ndarray:
         return self.kernel(self.points, self.points)
