@@ -289,11 +289,12 @@ def batch_stores(count: int, seed: int = 4, n_max: int = 16384):
         stress = tol <= 1e-8
         if not stress:
             try:
-                h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64)
+                h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, svd="range")
             except InfeasibleError:
                 stress = True
         if stress:
-            h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, stress_width=32)
+            h = W.build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, stress_width=32,
+                                       svd="range")
         natives.append(h.native)
         ks.append(int(rng.integers(1, 17)))
         shapes.append({"n": n, "dim": dim, "kernel": kern.__name__, "tol": tol,

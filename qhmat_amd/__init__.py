@@ -5,12 +5,31 @@ names, with ``device="cuda"`` served by hand-written gfx950 HIP kernels over a
 device-resident packed store.
 """
 
-from ._native import (Allocation, BlockInfo, NativeHMatrix, Partition, Rectangle,
-                      coefficient, cuda_built, spatial_partition)
-from ._native import allocate as allocate_bits
-from .matrix import HMatrix, InfeasibleError, StorageReport, memory_bytes
+from .blockwork import InfeasibleError
 
 __version__ = "0.1.0"
+
+# The names below load the native library on first use, not at package import:
+# the blockwise constructor's worker processes import qhmat_amd.blockwork and
+# must never load the HIP runtime.
+_NATIVE = {"Allocation", "BlockInfo", "NativeHMatrix", "Partition", "Rectangle", "coefficient",
+           "cuda_built", "spatial_partition"}
+_MATRIX = {"HMatrix", "StorageReport", "memory_bytes"}
+
+
+def __getattr__(name: str):
+    if name in _NATIVE or name == "allocate_bits":
+        from . import _native
+
+        value = _native.allocate if name == "allocate_bits" else getattr(_native, name)
+    elif name in _MATRIX:
+        from . import matrix
+
+        value = getattr(matrix, name)
+    else:
+        raise AttributeError(f"module 'qhmat_amd' has no attribute {name!r}")
+    globals()[name] = value
+    return value
 
 __all__ = [
     "Allocation",

@@ -574,16 +574,36 @@ __device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, i
     }
 }
 
+// sum_{s < count} p[s stride], added in order from zero, with the loads
+// issued four at a time (the same rounding as the plain loop).
+template <class T>
+__device__ __forceinline__ double ordered_sum(const T* p, std::uint64_t stride,
+                                              std::uint32_t count) {
+    double v = 0.0;
+    std::uint32_t s = 0;
+    for (; s + 4u <= count; s += 4u) {
+        const T a0 = p[s * stride], a1 = p[(s + 1u) * stride];
+        const T a2 = p[(s + 2u) * stride], a3 = p[(s + 3u) * stride];
+        v = (((v + double(a0)) + double(a1)) + double(a2)) + double(a3);
+    }
+    for (; s < count; ++s) v += double(p[s * stride]);
+    return v;
+}
+
 // A block's partial sums over its column tiles, summed in order into rows
 // in the components' virtual-row order (a split component's two rows both
-// carry the sum), one thread per (row, right-hand side).  Wave 0 lays out the
-// rows from the widths first, as build_vrows does.
+// carry the sum), one thread per (row, right-hand side): elements
+// [e0, e0 + 256) of the range's rows x this pass's right-hand sides.  Wave 0
+// lays out the rows from the widths first, as build_vrows does.
+constexpr std::uint32_t kZsumChunk = 256;
 template <bool FP32>
 __device__ __forceinline__ void zsum_group(const WideView& V, std::uint32_t g, int p0, int kc,
-                                           std::uint32_t* vrow) {
+                                           std::uint32_t e0, std::uint32_t* vrow) {
     using Real = real_t<FP32>;
     const ZSum zs = V.zsums[g];
     const std::uint32_t count = min(zs.count, std::uint32_t(kItemComponents));
+    const std::uint32_t w = min(std::uint32_t(kc), std::uint32_t(V.k) - std::uint32_t(p0));
+    if (e0 >= 2u * count * w) return;  // past every virtual row (at most two per component)
     __shared__ std::uint32_t nv_shared;
     if (threadIdx.x < 64) {
         const int lane = int(threadIdx.x);
@@ -595,33 +615,34 @@ __device__ __forceinline__ void zsum_group(const WideView& V, std::uint32_t g, i
     const std::uint32_t nv = nv_shared;
     const std::uint32_t k = std::uint32_t(V.k);
     Real* z = static_cast<Real*>(V.zw);
-    const std::uint32_t w = min(std::uint32_t(kc), k - std::uint32_t(p0));
-    for (std::uint32_t e = threadIdx.x; e < nv * w; e += blockDim.x) {
+    const std::uint32_t e = e0 + threadIdx.x;
+    if (e < nv * w) {
         const std::uint32_t v = e / w, p = std::uint32_t(p0) + e % w;
         const std::uint32_t j = vrow_comp(vrow[v]);
-        double s = 0.0;
-        for (std::uint32_t sg = 0; sg < zs.nseg; ++sg)
-            s += double(z[(std::uint64_t(zs.zrow) + std::uint64_t(sg) * zs.zstride + j) * k + p]);
+        const double s = ordered_sum(z + (std::uint64_t(zs.zrow) + j) * k + p,
+                                     std::uint64_t(zs.zstride) * k, zs.nseg);
         z[(std::uint64_t(zs.vzrow) + v) * k + p] = Real(s);
     }
 }
 
-// A tile's row sets summed in order into y, one thread per (row, right-hand side).
-__device__ __forceinline__ void sum_group(const WideView& V, std::uint32_t g) {
+// A tile's row sets summed in order into y: element `e` of the tile's rows
+// x right-hand sides (row-major), one per thread, 256 per workgroup.
+constexpr std::uint32_t kSumChunk = 256;
+__device__ __forceinline__ void sum_group(const WideView& V, std::uint32_t g, std::uint32_t e) {
     const WideSum ws = V.sums[g];
     const TileDev* __restrict__ T = V.tiles + ws.tile;
     const std::uint32_t R = min(T->rows, std::uint32_t(kTile));
     const std::uint32_t k = std::uint32_t(V.k);
     const double* sets = reinterpret_cast<const double*>(static_cast<const std::uint8_t*>(V.zw) +
                                                          V.z_rows * std::uint64_t(k) * sizeof(double));
-    for (std::uint32_t e = threadIdx.x; e < R * k; e += blockDim.x) {
-        const std::uint32_t i = e / k, p = e - i * k;
-        double v = 0.0;
-        for (std::uint32_t s = 0; s < ws.count; ++s)
-            v += sets[(std::uint64_t(ws.set0 + s) * kTile + i) * k + p];
-        V.y[std::int64_t(V.perm[T->row0 + i]) * V.ldy + p] = v;
-    }
+    if (e >= R * k) return;
+    const std::uint32_t i = e / k, p = e - i * k;
+    const double v = ordered_sum(sets + (std::uint64_t(ws.set0) * kTile + i) * k + p,
+                                 std::uint64_t(kTile) * k, ws.count);
+    V.y[std::int64_t(V.perm[T->row0 + i]) * V.ldy + p] = v;
 }
+// Workgroups per row-set sum for k right-hand sides.
+inline unsigned sum_chunks(int k) { return unsigned((kTile * std::uint32_t(k) + kSumChunk - 1) / kSumChunk); }
 
 // Waves per SIMD the register allocation must allow: pass R; pass L, whose
 // two fetch buffers grow with the right-hand sides per pass.
@@ -643,13 +664,17 @@ __global__ __launch_bounds__(64 * kWideWaves, left_occupancy<FP32>()) void wide_
     left_group<kLeftKC, FP32>(V, blockIdx.x, int(blockIdx.y) * kLeftKC, lds);
 }
 
-__global__ __launch_bounds__(256) void wide_sum(WideView V) { sum_group(V, blockIdx.x); }
+__global__ __launch_bounds__(kSumChunk) void wide_sum(WideView V) {
+    sum_group(V, blockIdx.x, blockIdx.y * kSumChunk + threadIdx.x);
+}
 
 template <bool FP32>
-__global__ __launch_bounds__(256) void wide_zsum(WideView V, int kc) {
+__global__ __launch_bounds__(kZsumChunk) void wide_zsum(WideView V, int kc) {
     __shared__ std::uint32_t vrow[128];
-    zsum_group<FP32>(V, blockIdx.x, int(blockIdx.y) * kc, kc, vrow);
+    zsum_group<FP32>(V, blockIdx.x, int(blockIdx.y) * kc, kc, blockIdx.z * kZsumChunk, vrow);
 }
+// Workgroups per range and pass: 128 virtual rows x kc right-hand sides.
+inline unsigned zsum_chunks(int kc) { return unsigned((128u * unsigned(kc) + kZsumChunk - 1) / kZsumChunk); }
 
 // Batched: member m of a launch owns global workgroups [first[m], first[m+1]).
 __device__ __forceinline__ std::uint32_t owner(const std::uint32_t* __restrict__ first,
@@ -696,7 +721,7 @@ __global__ __launch_bounds__(64 * kWideWaves, 3) void batch_wide_left(
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(256) void batch_wide_zsum(const WideView* __restrict__ members,
+__global__ __launch_bounds__(kZsumChunk) void batch_wide_zsum(const WideView* __restrict__ members,
                                                        const std::uint32_t* __restrict__ first,
                                                        std::uint32_t count, const double* x,
                                                        double* y, std::uint8_t* zw, int kc) {
@@ -705,16 +730,16 @@ __global__ __launch_bounds__(256) void batch_wide_zsum(const WideView* __restric
     __shared__ std::uint32_t vrow[128];
     const int p0 = int(blockIdx.y) * kc;
     if (p0 >= V.k) return;
-    zsum_group<FP32>(V, blockIdx.x - first[m], p0, kc, vrow);
+    zsum_group<FP32>(V, blockIdx.x - first[m], p0, kc, blockIdx.z * kZsumChunk, vrow);
 }
 
-__global__ __launch_bounds__(256) void batch_wide_sum(const WideView* __restrict__ members,
-                                                      const std::uint32_t* __restrict__ first,
-                                                      std::uint32_t count, const double* x,
-                                                      double* y, std::uint8_t* zw) {
+__global__ __launch_bounds__(kSumChunk) void batch_wide_sum(const WideView* __restrict__ members,
+                                                            const std::uint32_t* __restrict__ first,
+                                                            std::uint32_t count, const double* x,
+                                                            double* y, std::uint8_t* zw) {
     const std::uint32_t m = owner(first, count, blockIdx.x);
     const WideView V = bind(members, m, x, y, zw);
-    sum_group(V, blockIdx.x - first[m]);
+    sum_group(V, blockIdx.x - first[m], blockIdx.y * kSumChunk + threadIdx.x);
 }
 
 template <int KC, bool FP32>
@@ -727,7 +752,8 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
         hip_check(hipGetLastError(), "wide pass R launch");
     }
     if (c.zsums) {
-        hipLaunchKernelGGL((wide_zsum<FP32>), dim3(c.zsums, passes), dim3(256), 0, st, V, KC);
+        hipLaunchKernelGGL((wide_zsum<FP32>), dim3(c.zsums, passes, zsum_chunks(KC)),
+                           dim3(kZsumChunk), 0, st, V, KC);
         hip_check(hipGetLastError(), "wide partial-sum reduction launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
@@ -735,7 +761,7 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
                        dim3(64 * kWideWaves), 0, st, V);
     hip_check(hipGetLastError(), "wide pass L launch");
     if (c.sums) {
-        hipLaunchKernelGGL(wide_sum, dim3(c.sums), dim3(256), 0, st, V);
+        hipLaunchKernelGGL(wide_sum, dim3(c.sums, sum_chunks(V.k)), dim3(kSumChunk), 0, st, V);
         hip_check(hipGetLastError(), "wide row-set sum launch");
     }
     if (ev) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
@@ -752,8 +778,8 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
         hip_check(hipGetLastError(), "batched wide pass R launch");
     }
     if (c.zsums) {
-        hipLaunchKernelGGL((batch_wide_zsum<FP32>), dim3(c.zsums, passes), dim3(256), 0, st,
-                           c.members, c.first_zsum, n, x, y, zw, KC);
+        hipLaunchKernelGGL((batch_wide_zsum<FP32>), dim3(c.zsums, passes, zsum_chunks(KC)),
+                           dim3(kZsumChunk), 0, st, c.members, c.first_zsum, n, x, y, zw, KC);
         hip_check(hipGetLastError(), "batched wide partial-sum reduction launch");
     }
     hipLaunchKernelGGL((batch_wide_left<FP32>),
@@ -761,8 +787,8 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
                        dim3(64 * kWideWaves), 0, st, c.members, c.first_left, n, x, y, zw);
     hip_check(hipGetLastError(), "batched wide pass L launch");
     if (c.sums) {
-        hipLaunchKernelGGL(batch_wide_sum, dim3(c.sums), dim3(256), 0, st, c.members, c.first_sum,
-                           n, x, y, zw);
+        hipLaunchKernelGGL(batch_wide_sum, dim3(c.sums, sum_chunks(c.max_k)), dim3(kSumChunk), 0,
+                           st, c.members, c.first_sum, n, x, y, zw);
         hip_check(hipGetLastError(), "batched wide row-set sum launch");
     }
 }

@@ -19,10 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._native import NativeHMatrix, Rectangle, allocate, spatial_partition
-
-
-class InfeasibleError(ValueError):
-    """A non-diagonal block cannot meet its truncation or bit-allocation budget."""
+from .blockwork import InfeasibleError, truncate
 
 
 @dataclass(frozen=True)
@@ -103,38 +100,32 @@ def _square(matrix) -> np.ndarray:
 
 def factor_block(block: np.ndarray, rect: Rectangle, *, tolerance: float, widths: tuple,
                  rank_tolerance: float, max_rank: int | None, truncation_fraction: float,
-                 arithmetic_reserve: float):
+                 arithmetic_reserve: float, spectrum=None):
     """Rank, factors and widths of one off-diagonal block (matrix.py:213-245).
 
     Returns (u, sigma, v, bits, residual); rank 0 for an all-zero block.
     Raises InfeasibleError when the block cannot meet its budget.
+    ``spectrum`` as in blockwork.truncate (the randomized range finder for
+    large synthetic blocks; the full SVD when None).
     """
-    budget = tolerance * float(np.max(np.abs(block)))
-    m, n = block.shape
-    if budget == 0.0:
-        return np.zeros((m, 0)), np.empty(0), np.zeros((n, 0)), [], 0.0
-    u, sigma, vt = np.linalg.svd(block, full_matrices=False)
-    # tail[r] = Frobenius norm of the discarded part at rank r.
-    tail = np.sqrt(np.concatenate([np.cumsum(sigma[::-1] ** 2)[::-1], [0.0]]))
-    allowance = min(rank_tolerance * tail[0], truncation_fraction * budget)
-    cap = sigma.size if max_rank is None else min(max_rank, sigma.size)
-    fits = np.nonzero(tail[1:cap + 1] <= allowance)[0]
-    if fits.size == 0:
-        raise InfeasibleError(f"block ({rect.row0}:{rect.row1}, {rect.col0}:{rect.col1}) "
-                              "cannot satisfy the truncation allowance")
-    rank = min(int(fits[0]) + 1, int(np.count_nonzero(sigma > 0)))
-    sigma = sigma[:rank]
-    u = u[:, :rank]
-    v = vt[:rank, :].T
-    residual = float(np.max(np.abs(block - (u * sigma) @ v.T)))
-    quant_budget = budget * (1 - arithmetic_reserve) - residual
-    if residual > truncation_fraction * budget or quant_budget < 0:
-        raise InfeasibleError("measured truncation error exceeds the block allowance")
-    plan = allocate(sigma, widths, quant_budget, m, n)
+    box = (rect.row0, rect.row1, rect.col0, rect.col1)
+    u, sigma, v, residual, quant_budget = truncate(
+        block, box, tolerance=tolerance, rank_tolerance=rank_tolerance, max_rank=max_rank,
+        truncation_fraction=truncation_fraction, arithmetic_reserve=arithmetic_reserve,
+        spectrum=spectrum)
+    if quant_budget is None:
+        return u, sigma, v, [], residual
+    return u, sigma, v, allocate_widths(sigma, widths, quant_budget, block.shape, box), residual
+
+
+def allocate_widths(sigma, widths, quant_budget: float, shape, box) -> list:
+    """Bit widths of the kept components within the quantization budget
+    (native allocator); InfeasibleError when even the widest code misses it."""
+    plan = allocate(sigma, widths, quant_budget, shape[0], shape[1])
     if not plan.feasible:
-        raise InfeasibleError(f"block ({rect.row0}:{rect.row1}, {rect.col0}:{rect.col1}) "
+        raise InfeasibleError(f"block ({box[0]}:{box[1]}, {box[2]}:{box[3]}) "
                               "is infeasible even at the highest bit width")
-    return u, sigma, v, plan.bits, residual
+    return plan.bits
 
 
 class HMatrix:

@@ -4,8 +4,9 @@
 ``HMatrix.from_dense(kernel(points, points), points, ...)`` (same partition,
 same block values, same SVD and allocation per block) but evaluates each
 block from the points instead of forming the n x n dense matrix, and
-factors blocks on a thread pool.  Blocks are added in partition order, so the
-store is identical to the serial construction.
+factors blocks on a thread pool or, for large stores of the kernels below,
+on worker processes (qhmat_amd.blockwork).  Blocks are added in partition
+order, so the store is identical to the serial construction.
 
 Workloads (seeds fixed: geometry seed 0, right-hand sides seed 1):
   line_inverse   n=4096 uniform on [0,1], 1/(r+0.4), leaf 32, tol 1e-6, X 4096x4
@@ -16,6 +17,7 @@ Workloads (seeds fixed: geometry seed 0, right-hand sides seed 1):
 
 from __future__ import annotations
 
+import contextlib
 import math
 import os
 from concurrent.futures import ThreadPoolExecutor
@@ -24,47 +26,22 @@ from typing import Callable, Sequence
 
 import numpy as np
 
+from . import blockwork as B
 from ._native import NativeHMatrix, Rectangle, spatial_partition
-from .matrix import HMatrix, factor_block
+from .blockwork import (SingleLayer, distances, exponential, inverse_shifted, log_kernel,
+                        range_spectrum)
+from .matrix import HMatrix, allocate_widths
 
 FULL_MENU = (4, 8, 12, 16, 24, 32)
 
 
-# -- kernels: each maps (rows points, cols points) -> block values -----------------
-def distances(p: np.ndarray, q: np.ndarray) -> np.ndarray:
-    diff = p[:, None, :] - q[None, :, :]
-    return np.sqrt(np.sum(diff * diff, axis=2))
-
-
-def inverse_shifted(p, q):
-    """1 / (r + 0.4)."""
-    return 1.0 / (distances(p, q) + 0.4)
-
-
-def exponential(p, q):
-    """exp(-r / 0.2)."""
-    return np.exp(-distances(p, q) / 0.2)
-
-
-def log_kernel(p, q):
-    """log(r + 1e-2)."""
-    return np.log(distances(p, q) + 1e-2)
-
-
+# -- kernels (blockwork: NumPy only, importable by the worker processes) ------------
 KERNELS = {"inverse": inverse_shifted, "exp": exponential, "log": log_kernel}
 
 
 def single_layer(h: float) -> Callable:
     """1/(4 pi r) off the diagonal, 1/(4 pi 0.5 h) on it (point-panel self term)."""
-
-    def kernel(p, q):
-        r = distances(p, q)
-        with np.errstate(divide="ignore"):
-            out = 1.0 / (4.0 * math.pi * r)
-        out[r == 0.0] = 1.0 / (4.0 * math.pi * 0.5 * h)
-        return out
-
-    return kernel
+    return SingleLayer(h)
 
 
 # -- geometry -------------------------------------------------------------------------
@@ -88,9 +65,10 @@ def _blas_single_thread():
 
         return threadpool_limits(1)
     except Exception:  # pragma: no cover - threadpoolctl is optional
-        import contextlib
-
         return contextlib.nullcontext()
+
+
+PROCESS_MIN_ORDER = 2048  # smaller stores build on threads (no process start-up)
 
 
 def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: float,
@@ -100,7 +78,8 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                          arithmetic_reserve: float = 0.02, support_lower=None,
                          support_upper=None, workers: int | None = None,
                          progress: Callable[[int, int], None] | None = None,
-                         stress_width: int | None = None) -> HMatrix:
+                         stress_width: int | None = None, svd: str = "full",
+                         processes: bool | None = None) -> HMatrix:
     """Blockwise, parallel equivalent of from_dense(kernel(points, points), points, ...).
 
     ``progress(done, total)`` is called after every twentieth of the blocks.
@@ -111,7 +90,16 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
     needs (the SVD tail, as matrix.py:213-238 chooses it) and every component
     gets this width, added through add_low_rank (core.cpp:284-330), which
     checks no budget.  The store is labelled by its use: a stress case of the
-    widest codes, not a reference-constructible store."""
+    widest codes, not a reference-constructible store.
+
+    ``svd="range"`` factors blocks whose shorter side is at least 256 with
+    range_spectrum (randomized, exact residual bookkeeping) instead of the full
+    SVD the reference constructor takes: same tolerances and checks, ranks
+    equal or one-off, and each block's measured residual still bounds its
+    truncation; for the large synthetic stores whose full SVDs would take many
+    CPU-minutes."""
+    if svd not in ("full", "range"):
+        raise ValueError("svd must be 'full' or 'range'")
     p = np.ascontiguousarray(points, dtype=np.float64)
     lo = p if support_lower is None else np.asarray(support_lower, dtype=np.float64)
     hi = p if support_upper is None else np.asarray(support_upper, dtype=np.float64)
@@ -121,29 +109,32 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
     widths = tuple(bit_widths)
     n = p.shape[0]
 
-    def work(rect: Rectangle):
-        block = np.ascontiguousarray(kernel(clustered[rect.row0:rect.row1],
-                                            clustered[rect.col0:rect.col1]))
-        if rect.row0 == rect.col0 and rect.row1 == rect.col1:
-            return ("dense", block)
-        if stress_width is not None:
-            return ("lowrank", _tail_rank_factors(block, tolerance, rank_tolerance,
-                                                  truncation_fraction, stress_width))
-        return ("lowrank", factor_block(block, rect, tolerance=tolerance, widths=widths,
-                                        rank_tolerance=rank_tolerance, max_rank=max_rank,
-                                        truncation_fraction=truncation_fraction,
-                                        arithmetic_reserve=arithmetic_reserve))
-
+    params = dict(tolerance=tolerance, rank_tolerance=rank_tolerance, max_rank=max_rank,
+                  truncation_fraction=truncation_fraction,
+                  arithmetic_reserve=arithmetic_reserve, stress=stress_width is not None,
+                  svd=svd)
     native = NativeHMatrix(n, part.permutation)
-    workers = workers or min(32, os.cpu_count() or 1)
-    with _blas_single_thread(), ThreadPoolExecutor(max_workers=workers) as pool:
+    workers = workers or min(16, os.cpu_count() or 1)
+    blocks = list(part.blocks)
+    if processes is None:
+        # own kernels only: a worker imports the kernel by its module name
+        processes = (n >= PROCESS_MIN_ORDER and workers > 1
+                     and getattr(kernel, "__module__", "") == B.__name__)
+    job = B.make_job(clustered, kernel, params)
+    if processes:
+        # separate interpreters that import qhmat_amd.blockwork only (no
+        # HIP runtime, no GIL shared with this process)
+        pool, limits = B.WorkerPool(workers, job), contextlib.nullcontext()
+    else:
+        pool, limits = ThreadPoolExecutor(max_workers=workers), _blas_single_thread()
+    with limits, pool:
         # Bounded look-ahead keeps memory flat while preserving insertion order.
         pending = []
-        blocks = list(part.blocks)
         window = 4 * workers
         it = iter(blocks)
         for rect in it:
-            pending.append((rect, pool.submit(work, rect)))
+            pending.append((rect, (pool.submit(_box(rect)) if processes else
+                                    pool.submit(B.block_job, _box(rect), job))))
             if len(pending) >= window:
                 break
         total, done, step = len(blocks), 0, max(1, len(blocks) // 20)
@@ -156,31 +147,25 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
             if kind == "dense":
                 native.add_dense(rect, payload)
             else:
-                u, sigma, v, bits, residual = payload
+                u, sigma, v, residual, quant_budget = payload
+                if stress_width is not None:
+                    bits = [int(stress_width)] * sigma.size
+                elif quant_budget is None:
+                    bits = []
+                else:
+                    bits = allocate_widths(sigma, widths, quant_budget, (u.shape[0], v.shape[0]),
+                                           _box(rect))
                 native.add_low_rank(rect, u, sigma, v, bits, len(widths), residual)
             nxt = next(it, None)
             if nxt is not None:
-                pending.append((nxt, pool.submit(work, nxt)))
+                pending.append((nxt, (pool.submit(_box(nxt)) if processes else
+                                   pool.submit(B.block_job, _box(nxt), job))))
     native.finalize()
     return HMatrix(native, source_array_bytes=n * n * 8)
 
 
-def _tail_rank_factors(block, tolerance, rank_tolerance, truncation_fraction, width):
-    """(u, sigma, v, bits, residual) at the rank the truncation allowance
-    needs, every component `width` bits (no allocation budget)."""
-    budget = tolerance * float(np.max(np.abs(block)))
-    m, n = block.shape
-    if budget == 0.0:
-        return np.zeros((m, 0)), np.empty(0), np.zeros((n, 0)), [], 0.0
-    u, sigma, vt = np.linalg.svd(block, full_matrices=False)
-    tail = np.sqrt(np.concatenate([np.cumsum(sigma[::-1] ** 2)[::-1], [0.0]]))
-    allowance = min(rank_tolerance * tail[0], truncation_fraction * budget)
-    fits = np.nonzero(tail[1:] <= allowance)[0]
-    rank = int(fits[0]) + 1 if fits.size else sigma.size
-    rank = min(rank, int(np.count_nonzero(sigma > 0)))
-    u, sigma, v = u[:, :rank], sigma[:rank], vt[:rank, :].T
-    residual = float(np.max(np.abs(block - (u * sigma) @ v.T)))
-    return u, sigma, v, [int(width)] * rank, residual
+def _box(rect: Rectangle) -> tuple:
+    return (rect.row0, rect.row1, rect.col0, rect.col1)
 
 
 @dataclass

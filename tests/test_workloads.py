@@ -1,0 +1,88 @@
+"""The blockwise constructor of the synthetic workloads (host only).
+
+* worker processes and threads build byte-identical stores;
+* the worker module never loads the native library (so spawned workers never
+  touch the HIP runtime);
+* the randomized range finder's tail is the true discarded Frobenius norm of
+  the factors it returns, and its stores keep the full SVD's ranks;
+* the distance evaluation is bit-identical to a sum over the last axis.
+"""
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from qhmat_amd import blockwork as B
+from qhmat_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bytes(h) -> bytes:
+    d = tempfile.mkdtemp(prefix="qhmat-wl-")
+    path = os.path.join(d, "s.qhmat")
+    h.save(path)
+    try:
+        with open(path, "rb") as f:
+            return f.read()
+    finally:
+        os.remove(path)
+        os.rmdir(d)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_distances_bitwise(dim):
+    rng = np.random.default_rng(dim)
+    p, q = rng.uniform(size=(70, dim)), rng.uniform(size=(50, dim))
+    diff = p[:, None, :] - q[None, :, :]
+    assert np.array_equal(B.distances(p, q), np.sqrt(np.sum(diff * diff, axis=2)))
+
+
+def test_processes_match_threads():
+    pts = W.uniform_points(1536, 2, seed=7)
+    a = W.build_kernel_hmatrix(pts, W.exponential, tolerance=1e-6, leaf_size=64,
+                               processes=False, workers=4)
+    b = W.build_kernel_hmatrix(pts, W.exponential, tolerance=1e-6, leaf_size=64,
+                               processes=True, workers=3)
+    assert _bytes(a) == _bytes(b)
+
+
+def test_worker_module_loads_no_native_library():
+    code = ("import sys, qhmat_amd.blockwork as b\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(int('qhmat_amd._native' in sys.modules), int('libqhmat_hip' in maps),"
+            " int('amdhip64' in maps))\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True,
+                         text=True, timeout=120, check=True).stdout.split()
+    assert out == ["0", "0", "0"]
+
+
+def test_range_tail_is_discarded_norm():
+    rng = np.random.default_rng(3)
+    # a 400 x 300 matrix with geometric singular values
+    u = np.linalg.qr(rng.normal(size=(400, 300)))[0]
+    v = np.linalg.qr(rng.normal(size=(300, 300)))[0]
+    a = (u * 0.7 ** np.arange(300)) @ v.T
+    allowance = 1e-6 * np.linalg.norm(a)
+    uu, s, vt, tail = B.range_spectrum(a, allowance, seed=5)
+    assert s.size < 150  # the range finder stopped early, no full-SVD fallback
+    assert tail[-1] <= 0.5 * allowance
+    for r in (0, 5, 20, s.size // 2, s.size):
+        exact = np.linalg.norm(a - (uu[:, :r] * s[:r]) @ vt[:r])
+        assert tail[r] == pytest.approx(exact, rel=1e-6, abs=1e-12 * np.linalg.norm(a))
+    # the leading singular values are the matrix's own
+    np.testing.assert_allclose(s[:20], 0.7 ** np.arange(20), rtol=1e-10)
+
+
+def test_range_spectrum_keeps_full_ranks():
+    pts = W.uniform_points(2048, 1, seed=2)
+    kw = dict(tolerance=1e-6, leaf_size=64, processes=False, workers=4)
+    full = W.build_kernel_hmatrix(pts, W.log_kernel, svd="full", **kw)
+    fast = W.build_kernel_hmatrix(pts, W.log_kernel, svd="range", **kw)
+    assert [b.rank for b in full.native.blocks] == [b.rank for b in fast.native.blocks]
+    # each block's quantized factors stay inside its tolerance budget
+    assert fast.native.reconstruction_bound <= 1.01 * full.native.reconstruction_bound
