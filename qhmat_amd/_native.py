@@ -468,8 +468,10 @@ class NativeHMatrix:
 
     def set_rhs_block(self, k: int, kb: int, device: int = 0) -> None:
         """Applies of k right-hand sides on `device` run in passes of kb
-        (1, 2, 4, 8 or 16; 0 restores the default).  Bitwise the same result
-        for every kb; see qhmat_amd.tune for choosing it."""
+        right-hand sides: 1, 2 or 4 (vector-ALU passes) or 16, 32 or 64
+        (matrix-core passes); 0 restores the default.  The result is bitwise
+        the same for every kb of one of the two groups; see qhmat_amd.tune for
+        choosing it."""
         check(_lib.qh_store_set_rhs_block(self.device_store(device), int(k), int(kb)))
 
     def upload_copy(self, device: int = 0):

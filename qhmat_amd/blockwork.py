@@ -20,7 +20,9 @@ import numpy as np
 
 
 class InfeasibleError(ValueError):
-    """A non-diagonal block cannot meet its truncation or bit-allocation budget."""
+    """Raised when an off-diagonal block's error budget cannot be met: no
+    rank within the truncation allowance, or no width allocation within the
+    quantization budget."""
 
 
 # -- kernels: each maps (row points, column points) -> block values -----------------
@@ -290,17 +292,23 @@ class WorkerPool:
                                         stdout=subprocess.PIPE, env=env)
                        for _ in range(max(1, workers))]
         self._next = 0
+        self._unread = [0] * len(self._procs)  # answers not yet received, per worker
         for p in self._procs:
             _send(p.stdin, job)
 
     def submit(self, rect):
-        proc = self._procs[self._next]
+        i = self._next
         self._next = (self._next + 1) % len(self._procs)
-        _send(proc.stdin, rect)
-        return _Pending(proc)
+        _send(self._procs[i].stdin, rect)
+        self._unread[i] += 1
+        return _Pending(self, i)
 
     def close(self) -> None:
-        for p in self._procs:
+        """Ends the workers: at once for one with answers never read (the
+        caller stopped early, e.g. on an error), else when its input closes."""
+        for i, p in enumerate(self._procs):
+            if self._unread[i]:
+                p.kill()
             try:
                 p.stdin.close()
             except OSError:
@@ -311,6 +319,9 @@ class WorkerPool:
             except Exception:
                 p.kill()
                 p.wait()
+        for p in self._procs:
+            if p.stdout is not None:
+                p.stdout.close()
         self._procs = []
 
     def __enter__(self):
@@ -321,11 +332,12 @@ class WorkerPool:
 
 
 class _Pending:
-    def __init__(self, proc):
-        self._proc = proc
+    def __init__(self, pool: WorkerPool, index: int):
+        self._pool, self._index = pool, index
 
     def result(self):
-        status, value = _receive(self._proc.stdout)
+        status, value = _receive(self._pool._procs[self._index].stdout)
+        self._pool._unread[self._index] -= 1
         if status == "error":
             raise value
         return value

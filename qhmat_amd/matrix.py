@@ -24,10 +24,10 @@ from .blockwork import InfeasibleError, truncate
 
 @dataclass(frozen=True)
 class StorageReport:
-    """Dense baseline and stored-representation sizes, in bytes unless noted.
-
-    ``source_array_bytes`` is unavailable after loading a saved H-matrix.
-    """
+    """Sizes for comparing a store with its dense FP64 source: the n x n
+    FP64 array, the logical bit count of codes, scales and labels, and the
+    allocated and resident byte counts.  ``source_array_bytes`` is known only
+    for a store built in this process (None after ``load``)."""
 
     dense_fp64_bytes: int
     logical_bits: int
@@ -129,7 +129,9 @@ def allocate_widths(sigma, widths, quant_budget: float, shape, box) -> list:
 
 
 class HMatrix:
-    """A diagonal-dense, off-diagonal quantized hierarchical matrix.
+    """H-matrix whose diagonal blocks stay dense FP64 and whose other blocks
+    are truncated SVD factors stored as integer codes of per-component
+    widths.
 
     Construction uses NumPy's LAPACK-backed SVD; packing, allocation, storage
     accounting and the host apply are native; ``device="cuda"`` applies on the
@@ -148,7 +150,8 @@ class HMatrix:
                    rank_tolerance: float = 0.1, max_rank: int | None = None,
                    truncation_fraction: float = 0.25, arithmetic_reserve: float = 0.02,
                    support_lower=None, support_upper=None, verbose: bool = False) -> "HMatrix":
-        """Partition a dense matrix and quantize every non-diagonal block."""
+        """Cluster `points`, split `matrix` into dense diagonal and factored
+        off-diagonal blocks, and pack each off-diagonal block's factors."""
         a = _square(matrix)
         p = np.asarray(points, dtype=np.float64)
         if p.ndim != 2 or p.shape[0] != a.shape[0] or p.shape[1] == 0:
@@ -179,7 +182,8 @@ class HMatrix:
                        rank_tolerance: float = 0.1, max_rank: int | None = None,
                        truncation_fraction: float = 0.25, arithmetic_reserve: float = 0.02,
                        verbose: bool = False) -> "HMatrix":
-        """Build from user blocks expressed in permuted index coordinates."""
+        """Store the caller's block rectangles (rows and columns counted in the
+        order `permutation` gives) of `matrix`."""
         a = _square(matrix)
         if not np.isfinite(a).all():
             raise ValueError("matrix must be finite")
@@ -229,7 +233,8 @@ class HMatrix:
 
     @property
     def logical_bits(self) -> int:
-        """Packed payload, two FP64 scales, and direct precision labels."""
+        """Bits of the codes plus, per component, its two FP64 scales and its
+        width label."""
         return self._native.logical_bits
 
     @property
@@ -238,7 +243,7 @@ class HMatrix:
 
     @property
     def resident_bytes(self) -> int:
-        """Estimated owned host bytes; excludes the device copy."""
+        """Host bytes the store holds (estimate); the device copy is not counted."""
         return self._native.resident_bytes
 
     @property
@@ -257,7 +262,8 @@ class HMatrix:
 
     @property
     def reconstruction_bound(self) -> float:
-        """Max-norm bound for the stored matrix in exact arithmetic."""
+        """Upper bound on max |A - H| entrywise, H evaluated exactly (no
+        rounding in the apply)."""
         return self._native.reconstruction_bound
 
     # -- apply (matrix.py:321-333) ---------------------------------------------------
@@ -279,11 +285,11 @@ class HMatrix:
                                    precision=precision)
 
     def __matmul__(self, x):
-        """Apply on the host with the ``@`` operator."""
+        """``h @ x``: the host FP64 apply."""
         return self.matmul(x)
 
     def to_dense(self, *, device: str = "cpu") -> np.ndarray:
-        """Reconstruct a dense FP64 array; allocates ``8*n*n`` output bytes.
+        """The n x n FP64 matrix the store represents (an ``8*n*n``-byte array).
 
         ``device="cuda"`` forms it on the device from the resident store
         (bitwise the host result) and copies it back (matrix.py:309-311
@@ -310,7 +316,7 @@ def _is_device_tensor(x) -> bool:
 
 
 def memory_bytes(value) -> int:
-    """Array payload bytes or estimated owned host H-matrix bytes."""
+    """Bytes of a NumPy array's data, or the resident host bytes of an H-matrix."""
     if isinstance(value, np.ndarray):
         return int(value.nbytes)
     if isinstance(value, (HMatrix, NativeHMatrix)):

@@ -339,3 +339,76 @@ def poisson_load(server: Server, names: list[str], orders: dict[str, int], rate:
            "passes": server.passes}
     out.update({f"latency_{k}_ms": v for k, v in server.latency_ms().items()})
     return out
+
+
+def closed_loop_load(server: Server, names: list[str], orders: dict[str, int], rate: float,
+                     duration: float, ks=(1, 4, 16), session: int = 50, seed: int = 0) -> dict:
+    """Closed-loop GMRES-like clients: sessions start at Poisson times (rate /
+    session per second, so `rate` requests per second are offered while the
+    server keeps up); each session targets one store with one k and issues
+    its `session` requests one after another, the next as soon as the
+    previous result is back.  Returns the offered and achieved request rates,
+    the session count and the latency figures of the requests.
+    """
+    import torch
+
+    rng = np.random.default_rng(seed)
+    xs = {(nm, k): torch.randn(orders[nm], k, dtype=torch.float64, device=f"cuda:{server.device}")
+          for nm in names for k in ks}
+    torch.cuda.synchronize()
+    for key in xs:  # first launch of every shape, dropped from the statistics
+        server.submit(key[0], xs[key]).result()
+    server.reset_stats()
+    gaps = rng.exponential(session / rate, size=int(rate / session * duration * 1.5) + 4)
+    starts = np.cumsum(gaps)
+    starts = starts[starts < duration]
+    plans = [(names[int(rng.integers(len(names)))], int(ks[int(rng.integers(len(ks)))]))
+             for _ in starts]
+    lock = threading.Lock()
+    finished = threading.Event()
+    state = {"open": len(plans), "errors": []}
+
+    def run_session(name: str, k: int):
+        left = [session]
+
+        def step(fut=None):
+            if fut is not None:
+                err = fut.exception()
+                if err is not None:
+                    with lock:
+                        state["errors"].append(err)
+                    left[0] = 0
+            if left[0] == 0:
+                with lock:
+                    state["open"] -= 1
+                    if state["open"] == 0:
+                        finished.set()
+                return
+            left[0] -= 1
+            server.submit(name, xs[(name, k)]).add_done_callback(step)
+
+        step()
+
+    interval = sys.getswitchinterval()
+    sys.setswitchinterval(5e-5)
+    try:
+        t0 = time.perf_counter()
+        if not plans:
+            finished.set()
+        for at, (name, k) in zip(starts, plans):
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            run_session(name, k)
+        finished.wait()
+        elapsed = time.perf_counter() - t0
+    finally:
+        sys.setswitchinterval(interval)
+    if state["errors"]:
+        raise state["errors"][0]
+    done = server.completed
+    out = {"mode": "closed-loop sessions", "offered_rate": rate, "sessions": len(plans),
+           "session_requests": session, "requests": done, "seconds": round(elapsed, 4),
+           "throughput_rps": done / elapsed, "passes": server.passes}
+    out.update({f"latency_{k}_ms": v for k, v in server.latency_ms().items()})
+    return out

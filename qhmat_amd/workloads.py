@@ -216,3 +216,73 @@ def sphere_bem(n: int = 32768, tolerance: float = 1e-6) -> Workload:
     return Workload("sphere_bem", pts, single_layer(h), 64, tolerance, 64,
                     support_lower=pts - h, support_upper=pts + h,
                     note="mean spacing h = sqrt(4 pi / n)")
+
+
+def batch_workload(count: int = 64, seed: int = 4, n_min: int = 1024, n_max: int = 16384,
+                   progress: Callable[[str], None] | None = None):
+    """Stores of uneven order, kernel, dimension and tolerance for the batched
+    apply: n log-uniform in [n_min, n_max], kernels 1/(r+0.4), exp(-r/0.2),
+    log(r+1e-2) on 1-D, 2-D or 3-D uniform points, tolerances drawn from
+    {1e-4, 1e-6, 1e-8}, right-hand-side counts uniform in 1..16.  A store the
+    reference constructor refuses (its width budget infeasible: every 1e-8
+    store, and some larger 1e-6 ones) is built as a labelled stress store
+    (stress_width=32).  Returns (natives, ks, shapes)."""
+    from .blockwork import InfeasibleError
+
+    rng = np.random.default_rng(seed)
+    kernels = [inverse_shifted, exponential, log_kernel]
+    natives, ks, shapes = [], [], []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        dim = int(rng.integers(1, 4))
+        kern = kernels[int(rng.integers(0, 3))]
+        tol = (1e-4, 1e-6, 1e-8)[int(rng.integers(0, 3))]
+        pts = uniform_points(n, dim, seed=100 + i)
+        stress = tol <= 1e-8
+        if not stress:
+            try:
+                h = build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, svd="range")
+            except InfeasibleError:
+                stress = True
+        if stress:
+            h = build_kernel_hmatrix(pts, kern, tolerance=tol, leaf_size=64, stress_width=32,
+                                     svd="range")
+        natives.append(h.native)
+        ks.append(int(rng.integers(1, 17)))
+        shapes.append({"n": n, "dim": dim, "kernel": kern.__name__, "tol": tol,
+                       "stress": stress})
+        if progress is not None:
+            progress(f"store {i + 1}/{count}: n={n} {kern.__name__} {dim}-D tol {tol}"
+                     + (" (stress)" if stress else ""))
+    return natives, ks, shapes
+
+
+def serving_workload(count: int = 16, n_min: int = 2048, n_max: int = 32768,
+                     progress: Callable[[str], None] | None = None):
+    """Resident stores of the served load: orders spaced geometrically from
+    n_min to n_max, kernels 1/(r+0.4), exp(-r/0.2), log(r+1e-2) in turn on
+    1-D, 2-D and 3-D uniform points, tolerance 1e-6, leaf 64 (a store the
+    reference constructor refuses at 1e-6 is built as a labelled stress
+    store).  Returns (name -> native, shapes)."""
+    from .blockwork import InfeasibleError
+
+    kernels = [inverse_shifted, exponential, log_kernel]
+    natives, shapes = {}, []
+    for i in range(count):
+        n = int(round(n_min * (n_max / n_min) ** (i / max(count - 1, 1))))
+        kern, dim = kernels[i % 3], 1 + (i // 3) % 3
+        pts = uniform_points(n, dim, seed=300 + i)
+        stress = False
+        try:
+            h = build_kernel_hmatrix(pts, kern, tolerance=1e-6, leaf_size=64, svd="range")
+        except InfeasibleError:  # labelled stress store, as in batch_workload
+            stress = True
+            h = build_kernel_hmatrix(pts, kern, tolerance=1e-6, leaf_size=64, stress_width=32,
+                                     svd="range")
+        name = f"{kern.__name__}_{dim}d_{n}"
+        natives[name] = h.native
+        shapes.append({"name": name, "n": n, "dim": dim, "kernel": kern.__name__,
+                       "stress": stress})
+        if progress is not None:
+            progress(f"store {i + 1}/{count}: {name}" + (" (stress)" if stress else ""))
+    return natives, shapes

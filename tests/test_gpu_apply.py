@@ -172,3 +172,49 @@ def test_error_bound_against_source(line_inverse):
     for precision in ("fp64", "fp32split"):
         err = np.abs(native.matmul(x, True, precision=precision) - exact).max(axis=0)
         assert np.all(err <= bound + 1e-9)
+
+
+def test_concurrent_host_threads(stores):
+    """Several host threads apply one store at once (the reference releases
+    the interpreter lock in matmul, bindings.cpp:164) while another retunes
+    its passes: every result matches the oracle."""
+    import threading
+
+    native, orc = stores["ragged_square"]
+    rng = np.random.default_rng(11)
+    cases = [(k, prec, rng.normal(size=(native.order, k)))
+             for k in (1, 3, 16, 33) for prec in ("fp64", "fp32split")]
+    want = {i: orc.matmul(x) for i, (_, _, x) in enumerate(cases)}
+    errors, stop = [], threading.Event()
+
+    def worker(seed):
+        order = np.random.default_rng(seed).permutation(len(cases))
+        try:
+            for _ in range(3):
+                for i in order:
+                    k, prec, x = cases[i]
+                    got = native.matmul(x, True, precision=prec)
+                    H.same_result(got, want[i], prec)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    def retune():
+        kbs = [1, 2, 4, 16, 32, 0]
+        i = 0
+        while not stop.is_set():
+            native.set_rhs_block(33, kbs[i % len(kbs)])
+            native.set_rhs_block(16, kbs[(i + 1) % len(kbs)])
+            i += 1
+        native.set_rhs_block(33, 0)
+        native.set_rhs_block(16, 0)
+
+    threads = [threading.Thread(target=worker, args=(s,)) for s in range(6)]
+    tuner = threading.Thread(target=retune)
+    tuner.start()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=300)
+    stop.set()
+    tuner.join(timeout=60)
+    assert not errors, errors[0]

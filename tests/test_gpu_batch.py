@@ -1,6 +1,7 @@
 """Batched apply of uneven stores and its HIP-graph replay: every member's
-result must be bitwise the result of applying that store alone, and every
-graph replay must reproduce the eager result."""
+result must be bitwise the result of applying that store alone and match
+the oracle within the precision's tolerance, and every graph replay must
+reproduce the eager result."""
 
 import numpy as np
 import pytest
@@ -46,6 +47,8 @@ def test_batch_matches_single_applies(members, precision):
         alone = native.apply_device(xi.contiguous(), precision=precision)
         torch.cuda.synchronize()
         assert torch.equal(yi, alone)
+        # and the oracle's apply of the member's own x
+        H.same_result(yi.cpu().numpy(), H.oracle_of(native).matmul(xi.cpu().numpy()), precision)
 
 
 def test_batch_graph_replays(members):

@@ -1,12 +1,19 @@
 """Dense reconstruction on the device (qh_reconstruct, HMatrix.to_dense with
-device="cuda"): bitwise the host reconstruction, which matches the reference
-core's (tests/test_oracle.py); the reference's own check that the apply equals
-the reconstructed matrix times X (tests/test_core.py:192) on the device."""
+device="cuda"): bitwise the oracle's reconstruction (orc_reconstruct, the
+reference core's loop order: sum over components of (code_l lscale)
+(code_r rscale), tests/test_oracle.py) and the reference-written golden
+dense matrices; the reference's own check that the apply equals the
+reconstructed matrix times X (tests/test_core.py:192) on the device."""
+
+from pathlib import Path
 
 import numpy as np
 import pytest
 
+import qhmat_amd as qhmat
 from tests import helpers as H
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 pytestmark = pytest.mark.gpu
 
@@ -30,10 +37,22 @@ BUILDERS = {
 
 
 @pytest.mark.parametrize("name", list(BUILDERS))
-def test_bitwise_host_reconstruction(torch, name):
+def test_bitwise_oracle_reconstruction(torch, name):
     native = BUILDERS[name]()
     got = native.reconstruct_device().cpu().numpy()
+    np.testing.assert_array_equal(got, H.oracle_of(native).reconstruct())
     np.testing.assert_array_equal(got, native.reconstruct())
+
+
+GOLDEN_DENSE = sorted(p.stem for p in GOLDEN.glob("*.qhmat")
+                      if "dense" in np.load(GOLDEN / f"{p.stem}.npz"))
+
+
+@pytest.mark.parametrize("name", GOLDEN_DENSE)
+def test_bitwise_golden_dense(torch, name):
+    h = qhmat.HMatrix.load(GOLDEN / f"{name}.qhmat")
+    dense = np.load(GOLDEN / f"{name}.npz")["dense"]
+    np.testing.assert_array_equal(h.native.reconstruct_device().cpu().numpy(), dense)
 
 
 def test_strided_output_and_apply_consistency(torch):

@@ -1,7 +1,7 @@
 """Serving loop: admission against the HBM budget, continuous batching, and
-each request's result the same alone and in a mix: bitwise while the merged
+each request's result the same alone and in a mix (bitwise while the merged
 pass stays on the request's arithmetic path, within the precision's tolerance
-otherwise."""
+otherwise) and equal to the oracle's apply within the precision's tolerance."""
 
 import numpy as np
 import pytest
@@ -59,6 +59,10 @@ def test_results_same_alone_and_in_mix(natives, precision):
         assert server.passes < len(plan)  # requests were merged into shared passes
     for got, want in zip(results, alone):
         H.same_result(got.cpu().numpy(), want.cpu().numpy(), precision)
+    # every served result against the oracle's apply of its own x
+    oracles = {name: H.oracle_of(native) for name, native in natives.items()}
+    for (name, _), x, got in zip(plan, xs, results):
+        H.same_result(got.cpu().numpy(), oracles[name].matmul(x.cpu().numpy()), precision)
     lat = server.latency_ms()
     assert lat["p50"] is not None and lat["p99"] >= lat["p50"]
 

@@ -86,3 +86,18 @@ def test_range_spectrum_keeps_full_ranks():
     assert [b.rank for b in full.native.blocks] == [b.rank for b in fast.native.blocks]
     # each block's quantized factors stay inside its tolerance budget
     assert fast.native.reconstruction_bound <= 1.01 * full.native.reconstruction_bound
+
+
+def test_worker_error_raises_promptly():
+    """An infeasible block raised in a worker reaches the caller, and the
+    workers with answers still unread are ended at once."""
+    import time
+
+    from qhmat_amd import InfeasibleError
+
+    pts = W.uniform_points(2500, 2, seed=1)
+    t = time.time()
+    with pytest.raises(InfeasibleError):
+        W.build_kernel_hmatrix(pts, W.exponential, tolerance=1e-8, leaf_size=64,
+                               processes=True, workers=3)
+    assert time.time() - t < 25
