@@ -592,18 +592,15 @@ __device__ __forceinline__ double ordered_sum(const T* p, std::uint64_t stride,
 
 // A block's partial sums over its column tiles, summed in order into rows
 // in the components' virtual-row order (a split component's two rows both
-// carry the sum), one thread per (row, right-hand side): elements
-// [e0, e0 + 256) of the range's rows x this pass's right-hand sides.  Wave 0
-// lays out the rows from the widths first, as build_vrows does.
-constexpr std::uint32_t kZsumChunk = 256;
+// carry the sum), one thread per (row, right-hand side).  Wave 0 lays out the
+// rows from the widths first, as build_vrows does.
 template <bool FP32>
 __device__ __forceinline__ void zsum_group(const WideView& V, std::uint32_t g, int p0, int kc,
-                                           std::uint32_t e0, std::uint32_t* vrow) {
+                                           std::uint32_t* vrow) {
     using Real = real_t<FP32>;
     const ZSum zs = V.zsums[g];
     const std::uint32_t count = min(zs.count, std::uint32_t(kItemComponents));
     const std::uint32_t w = min(std::uint32_t(kc), std::uint32_t(V.k) - std::uint32_t(p0));
-    if (e0 >= 2u * count * w) return;  // past every virtual row (at most two per component)
     __shared__ std::uint32_t nv_shared;
     if (threadIdx.x < 64) {
         const int lane = int(threadIdx.x);
@@ -615,8 +612,7 @@ __device__ __forceinline__ void zsum_group(const WideView& V, std::uint32_t g, i
     const std::uint32_t nv = nv_shared;
     const std::uint32_t k = std::uint32_t(V.k);
     Real* z = static_cast<Real*>(V.zw);
-    const std::uint32_t e = e0 + threadIdx.x;
-    if (e < nv * w) {
+    for (std::uint32_t e = threadIdx.x; e < nv * w; e += blockDim.x) {
         const std::uint32_t v = e / w, p = std::uint32_t(p0) + e % w;
         const std::uint32_t j = vrow_comp(vrow[v]);
         const double s = ordered_sum(z + (std::uint64_t(zs.zrow) + j) * k + p,
@@ -648,9 +644,10 @@ inline unsigned sum_chunks(int k) { return unsigned((kTile * std::uint32_t(k) + 
 // two fetch buffers grow with the right-hand sides per pass.
 template <bool FP32> constexpr int wide_occupancy() { return FP32 ? 4 : 3; }
 // Pass L runs 16 right-hand sides per launch slice whatever pass R's width:
-// its waves hold FP64 sums of all 64 rows, 32 registers per 16 columns.
-constexpr int kLeftKC = 16;
-template <bool FP32> constexpr int left_occupancy() { return FP32 ? 4 : 3; }
+// its waves hold FP64 sums of all 64 rows, 32 registers per 16 columns (32
+// per slice spill at three waves per SIMD).
+template <bool FP32> constexpr int left_kc(int) { return 16; }
+template <int LKC, bool FP32> constexpr int left_occupancy() { return FP32 ? 4 : 3; }
 
 template <int KC, bool FP32>
 __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_right(WideView V) {
@@ -658,10 +655,10 @@ __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_
     right_group<KC, FP32>(V, blockIdx.x, int(blockIdx.y) * KC, lds);
 }
 
-template <bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, left_occupancy<FP32>()) void wide_left(WideView V) {
-    __shared__ __align__(16) std::uint8_t lds[LeftShape<kLeftKC, FP32>::kBytes];
-    left_group<kLeftKC, FP32>(V, blockIdx.x, int(blockIdx.y) * kLeftKC, lds);
+template <int LKC, bool FP32>
+__global__ __launch_bounds__(64 * kWideWaves, (left_occupancy<LKC, FP32>())) void wide_left(WideView V) {
+    __shared__ __align__(16) std::uint8_t lds[LeftShape<LKC, FP32>::kBytes];
+    left_group<LKC, FP32>(V, blockIdx.x, int(blockIdx.y) * LKC, lds);
 }
 
 __global__ __launch_bounds__(kSumChunk) void wide_sum(WideView V) {
@@ -669,12 +666,10 @@ __global__ __launch_bounds__(kSumChunk) void wide_sum(WideView V) {
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(kZsumChunk) void wide_zsum(WideView V, int kc) {
+__global__ __launch_bounds__(256) void wide_zsum(WideView V, int kc) {
     __shared__ std::uint32_t vrow[128];
-    zsum_group<FP32>(V, blockIdx.x, int(blockIdx.y) * kc, kc, blockIdx.z * kZsumChunk, vrow);
+    zsum_group<FP32>(V, blockIdx.x, int(blockIdx.y) * kc, kc, vrow);
 }
-// Workgroups per range and pass: 128 virtual rows x kc right-hand sides.
-inline unsigned zsum_chunks(int kc) { return unsigned((128u * unsigned(kc) + kZsumChunk - 1) / kZsumChunk); }
 
 // Batched: member m of a launch owns global workgroups [first[m], first[m+1]).
 __device__ __forceinline__ std::uint32_t owner(const std::uint32_t* __restrict__ first,
@@ -708,20 +703,20 @@ __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void batch
     right_group<KC, FP32>(V, blockIdx.x - first[m], p0, lds);
 }
 
-template <bool FP32>
+template <int LKC, bool FP32>
 __global__ __launch_bounds__(64 * kWideWaves, 3) void batch_wide_left(
     const WideView* __restrict__ members, const std::uint32_t* __restrict__ first,
     std::uint32_t count, const double* x, double* y, std::uint8_t* zw) {
-    __shared__ __align__(16) std::uint8_t lds[LeftShape<kLeftKC, FP32>::kBytes];
+    __shared__ __align__(16) std::uint8_t lds[LeftShape<LKC, FP32>::kBytes];
     const std::uint32_t m = owner(first, count, blockIdx.x);
     const WideView V = bind(members, m, x, y, zw);
-    const int p0 = int(blockIdx.y) * kLeftKC;
+    const int p0 = int(blockIdx.y) * LKC;
     if (p0 >= V.k) return;
-    left_group<kLeftKC, FP32>(V, blockIdx.x - first[m], p0, lds);
+    left_group<LKC, FP32>(V, blockIdx.x - first[m], p0, lds);
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(kZsumChunk) void batch_wide_zsum(const WideView* __restrict__ members,
+__global__ __launch_bounds__(256) void batch_wide_zsum(const WideView* __restrict__ members,
                                                        const std::uint32_t* __restrict__ first,
                                                        std::uint32_t count, const double* x,
                                                        double* y, std::uint8_t* zw, int kc) {
@@ -730,7 +725,7 @@ __global__ __launch_bounds__(kZsumChunk) void batch_wide_zsum(const WideView* __
     __shared__ std::uint32_t vrow[128];
     const int p0 = int(blockIdx.y) * kc;
     if (p0 >= V.k) return;
-    zsum_group<FP32>(V, blockIdx.x - first[m], p0, kc, blockIdx.z * kZsumChunk, vrow);
+    zsum_group<FP32>(V, blockIdx.x - first[m], p0, kc, vrow);
 }
 
 __global__ __launch_bounds__(kSumChunk) void batch_wide_sum(const WideView* __restrict__ members,
@@ -752,12 +747,12 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
         hip_check(hipGetLastError(), "wide pass R launch");
     }
     if (c.zsums) {
-        hipLaunchKernelGGL((wide_zsum<FP32>), dim3(c.zsums, passes, zsum_chunks(KC)),
-                           dim3(kZsumChunk), 0, st, V, KC);
+        hipLaunchKernelGGL((wide_zsum<FP32>), dim3(c.zsums, passes), dim3(256), 0, st, V, KC);
         hip_check(hipGetLastError(), "wide partial-sum reduction launch");
     }
     if (ev) hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
-    hipLaunchKernelGGL((wide_left<FP32>), dim3(c.left_groups, unsigned((V.k + kLeftKC - 1) / kLeftKC)),
+    constexpr int LKC = left_kc<FP32>(KC);
+    hipLaunchKernelGGL((wide_left<LKC, FP32>), dim3(c.left_groups, unsigned((V.k + LKC - 1) / LKC)),
                        dim3(64 * kWideWaves), 0, st, V);
     hip_check(hipGetLastError(), "wide pass L launch");
     if (c.sums) {
@@ -778,12 +773,13 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
         hip_check(hipGetLastError(), "batched wide pass R launch");
     }
     if (c.zsums) {
-        hipLaunchKernelGGL((batch_wide_zsum<FP32>), dim3(c.zsums, passes, zsum_chunks(KC)),
-                           dim3(kZsumChunk), 0, st, c.members, c.first_zsum, n, x, y, zw, KC);
+        hipLaunchKernelGGL((batch_wide_zsum<FP32>), dim3(c.zsums, passes), dim3(256), 0, st,
+                           c.members, c.first_zsum, n, x, y, zw, KC);
         hip_check(hipGetLastError(), "batched wide partial-sum reduction launch");
     }
-    hipLaunchKernelGGL((batch_wide_left<FP32>),
-                       dim3(c.left_groups, unsigned((c.max_k + kLeftKC - 1) / kLeftKC)),
+    constexpr int LKC = left_kc<FP32>(KC);
+    hipLaunchKernelGGL((batch_wide_left<LKC, FP32>),
+                       dim3(c.left_groups, unsigned((c.max_k + LKC - 1) / LKC)),
                        dim3(64 * kWideWaves), 0, st, c.members, c.first_left, n, x, y, zw);
     hip_check(hipGetLastError(), "batched wide pass L launch");
     if (c.sums) {
