@@ -391,7 +391,9 @@ def main() -> int:
                     help="p99 latency bound the served load is reported against")
     ap.add_argument("--harness", action="store_true",
                     help="also time the application-harness methods (dense GEMM, batched GEMMs on "
-                         "decoded factors, the fused apply) on the same store at k = 1 and 32")
+                         "unquantized, per-application-decoded and predecoded factors, the fused "
+                         "apply) at k = 1, 8, 32 on the headline store and on the reference "
+                         "suite's 2-D 1/r n=2048 and cube BEM n=1536 cases at tol 0.01")
     args = ap.parse_args()
     if 1 not in args.rhs:
         args.rhs = [1] + args.rhs  # the headline value is quoted at k=1
@@ -453,11 +455,18 @@ def main() -> int:
         from qhmat_amd.harness import Harness
 
         log("harness: reconstructing and decoding the store")
-        hn = Harness(head.native)
-        harness = {f"k{k}": hn.run(k) for k in (1, 32)}
-        for kk, res in harness.items():
-            log(f"harness {kk}: " + ", ".join(f"{m} {v['ms'] * 1e3:.1f} us" for m, v in res.items()))
-        del hn
+        harness = {}
+        cases = [("square_exp_16384", head.native, None)]
+        for case in (W.suite_inverse_distance_2d(), W.suite_cube_bem()):
+            built = case.build(1e-2)
+            cases.append((f"{case.name}_tol0.01", built.native, built.factors))
+        for name, native, factors in cases:
+            hn = Harness(native, factors=factors)
+            harness[name] = {"n": native.order, **{f"k{k}": hn.run(k) for k in (1, 8, 32)}}
+            for kk in ("k1", "k32"):
+                log(f"harness {name} {kk}: " + ", ".join(
+                    f"{m} {v['ms'] * 1e3:.1f} us" for m, v in harness[name][kk].items()))
+            del hn
 
     cpu = None
     if not args.no_cpu:

@@ -227,6 +227,25 @@ qh_status qh_reconstruct(const qh_store *store, double *out, int64_t ld, void *s
 qh_status qh_decode_codes(const qh_store *store, uint64_t block, int side, int32_t *out,
                           void *stream);
 
+/* Decoded FP32 factors for batched-GEMM applications of a resident store
+ * (the application harness's methods that decode on every application:
+ * experiments/cuda/application.cu:189-222, `adaptive_unpacked_batched`, whose
+ * unpack_jobs kernel this replaces; no counterpart in the library's API).
+ * A plan lists factor sides: block `blocks[i]` (insertion index, a low-rank
+ * block of rank > 0), side `sides[i]` (0: the left factor as rows x rank
+ * row-major; 1: the right factor transposed, rank x cols row-major), written
+ * at `offsets[i]` floats into an output of `out_floats` floats.  Each value
+ * is float(code x that side's component scale).  qh_factor_plan_decode
+ * decodes every listed side from the resident words in one stream-ordered
+ * launch; the plan holds a task table in device memory until destroyed
+ * (destroy it before its store). */
+typedef struct qh_factor_plan qh_factor_plan;
+qh_status qh_factor_plan_create(const qh_store *store, const uint64_t *blocks, const int32_t *sides,
+                                const uint64_t *offsets, uint64_t count, uint64_t out_floats,
+                                qh_factor_plan **plan);
+qh_status qh_factor_plan_decode(const qh_factor_plan *plan, float *out, void *stream);
+qh_status qh_factor_plan_destroy(qh_factor_plan *plan);
+
 /* ---- host store (NativeHMatrix, bindings.cpp:124-198) ------------------- */
 
 qh_status qh_host_create(uint64_t order, const uint64_t *perm, qh_host **out);

@@ -52,6 +52,43 @@ class StoreDesc(C.Structure):
     ]
 
 
+class FactorPlan:
+    """Decodes listed factor sides of a resident store into one FP32 device
+    buffer per call (qh_factor_plan_*): side 0 of a block as rows x rank,
+    side 1 as rank x cols (the right factor transposed), row-major, each at
+    its offset (in floats)."""
+
+    def __init__(self, native, blocks, sides, offsets, out_floats: int, device: int = 0):
+        b = np.ascontiguousarray(blocks, dtype=np.uint64)
+        sd = np.ascontiguousarray(sides, dtype=np.int32)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        if not (b.shape == sd.shape == off.shape) or b.ndim != 1:
+            raise ValueError("blocks, sides and offsets must be one-dimensional and alike")
+        self._native = native  # the store outlives the plan
+        self.out_floats = int(out_floats)
+        h = C.c_void_p()
+        check(_lib.qh_factor_plan_create(native.device_store(device), _ptr(b), _ptr(sd), _ptr(off),
+                                         b.size, self.out_floats, C.byref(h)))
+        self._h = h
+
+    def decode(self, out, stream=None) -> None:
+        """Writes the factors into `out`, an FP32 device tensor of at least
+        out_floats elements, on `stream` (default: torch's current stream)."""
+        import torch
+
+        if out.dtype != torch.float32 or not out.is_cuda or out.numel() < self.out_floats \
+                or not out.is_contiguous():
+            raise ValueError("output must be a contiguous FP32 device tensor of out_floats elements")
+        st = torch.cuda.current_stream() if stream is None else stream
+        check(_lib.qh_factor_plan_decode(self._h, C.c_void_p(out.data_ptr()),
+                                         C.c_void_p(st.cuda_stream)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.qh_factor_plan_destroy(self._h)
+            self._h = None
+
+
 class StoreStats(C.Structure):
     _fields_ = [
         ("order", _u64), ("nblocks", _u64), ("ndense", _u64), ("nlowrank", _u64),
@@ -112,6 +149,9 @@ _sig("qh_apply_noalloc", _p, _p, _i64, _i64, _p, _i64, C.c_int, _p)
 _sig("qh_noalloc_scratch_bytes", C.POINTER(_u64))
 _sig("qh_decode_codes", _p, _u64, C.c_int, _p, _p)
 _sig("qh_reconstruct", _p, _p, _i64, _p)
+_sig("qh_factor_plan_create", _p, _p, _p, _p, _u64, _u64, C.POINTER(_p))
+_sig("qh_factor_plan_decode", _p, _p, _p)
+_sig("qh_factor_plan_destroy", _p)
 _sig("qh_plan_emulate", C.POINTER(StoreDesc), _p, _i64, _p)
 _sig("qh_plan_emulate_wide", C.POINTER(StoreDesc), _p, _i64, _p, C.c_int)
 _sig("qh_store_set_rhs_block", _p, _i64, C.c_int)
@@ -153,6 +193,11 @@ def check(status: int) -> None:
     if status == QH_ENOMEM:
         raise MemoryError(message)
     raise RuntimeError(message)
+
+
+def version() -> str:
+    """The native library's version string (qh_version)."""
+    return _lib.qh_version().decode()
 
 
 def lib() -> C.CDLL:

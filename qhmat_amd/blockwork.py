@@ -66,6 +66,59 @@ class SingleLayer:
         return out
 
 
+def inverse_distance(p, q):
+    """1 / r between distinct points, 0 where r = 0 (the omitted self term of
+    the point-kernel suite cases)."""
+    r = distances(p, q)
+    with np.errstate(divide="ignore"):
+        out = 1.0 / r
+    out[r == 0.0] = 0.0
+    return out
+
+
+def _rect_primitive(u, v, z):
+    """F(u, v) with d2F/du dv = 1 / sqrt(u^2 + v^2 + z^2) (z >= 0):
+    u asinh(v / sqrt(u^2 + z^2)) + v asinh(u / sqrt(v^2 + z^2))
+    - z atan(u v / (z sqrt(u^2 + v^2 + z^2))), each term taken as its limit 0
+    where its argument is undefined (u = z = 0, v = z = 0, or z = 0)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ru = np.sqrt(u * u + z * z)
+        rv = np.sqrt(v * v + z * z)
+        r = np.sqrt(u * u + v * v + z * z)
+        t1 = np.where(ru > 0, u * np.arcsinh(v / np.where(ru > 0, ru, 1.0)), 0.0)
+        t2 = np.where(rv > 0, v * np.arcsinh(u / np.where(rv > 0, rv, 1.0)), 0.0)
+        t3 = np.where(z > 0, z * np.arctan(u * v / np.where(z > 0, z * r, 1.0)), 0.0)
+    return t1 + t2 - t3
+
+
+class RectangleSingleLayer:
+    """Collocation single layer on axis-aligned rectangular panels: entry
+    (i, j) = (1 / 4 pi) times the integral of 1 / |x_i - y| over panel j, x_i
+    the centre of panel i, by the closed form of that integral (corners of
+    _rect_primitive), the self terms included.  Rows of the kernel data:
+    centre (3), half-widths (3, zero along the normal), normal axis (1)."""
+
+    def __call__(self, p, q):
+        out = np.empty((p.shape[0], q.shape[0]))
+        normals = q[:, 6].astype(np.int64)
+        for axis in range(3):
+            cols = np.flatnonzero(normals == axis)
+            if cols.size == 0:
+                continue
+            a, b = [d for d in range(3) if d != axis]
+            src = q[cols]
+            z = np.abs(src[None, :, axis] - p[:, None, axis])
+            du = src[None, :, a] - p[:, None, a]
+            dv = src[None, :, b] - p[:, None, b]
+            hu, hv = src[None, :, 3 + a], src[None, :, 3 + b]
+            total = np.zeros_like(z)
+            for su in (-1.0, 1.0):
+                for sv in (-1.0, 1.0):
+                    total += su * sv * _rect_primitive(du + su * hu, dv + sv * hv, z)
+            out[:, cols] = total / (4.0 * math.pi)
+        return out
+
+
 # -- spectra ----------------------------------------------------------------------
 RANGE_MIN_SIDE = 256  # blocks with a shorter side take the full SVD
 RANGE_STEP = 32

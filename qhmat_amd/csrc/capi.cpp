@@ -298,6 +298,43 @@ qh_status qh_decode_codes(const qh_store* store, uint64_t block, int side, int32
     });
 }
 
+struct qh_factor_plan {
+    qhamd::FactorPlan* plan = nullptr;
+};
+
+qh_status qh_factor_plan_create(const qh_store* store, const uint64_t* blocks, const int32_t* sides,
+                                const uint64_t* offsets, uint64_t count, uint64_t out_floats,
+                                qh_factor_plan** plan) {
+    return guarded([&] {
+        need(store, "store");
+        need(plan, "output handle");
+        *plan = nullptr;
+        std::vector<int> sd(sides, sides + (sides ? count : 0));
+        auto* h = new qh_factor_plan;
+        try {
+            h->plan = qhamd::create_factor_plan(*store->dev, blocks, sd.data(), offsets, count,
+                                                out_floats);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
+qh_status qh_factor_plan_decode(const qh_factor_plan* plan, float* out, void* stream) {
+    return guarded([&] {
+        need(plan, "plan");
+        qhamd::factor_plan_decode(*plan->plan, out, stream);
+    });
+}
+
+qh_status qh_factor_plan_destroy(qh_factor_plan* plan) {
+    if (plan) qhamd::destroy_factor_plan(plan->plan);
+    delete plan;
+    return QH_OK;
+}
+
 qh_status qh_host_create(uint64_t order, const uint64_t* perm, qh_host** out) {
     return guarded([&] {
         need(out, "output handle");

@@ -920,6 +920,29 @@ __global__ void decode_side(const std::uint32_t* __restrict__ words,
     }
 }
 
+// One chunk of a factor side per workgroup, decoded to FP32 code x scale:
+// side 0 at out[offset + row rank + j], side 1 at out[offset + j rows + row].
+__global__ __launch_bounds__(256) void decode_factors(const FactorTask* __restrict__ tasks,
+                                                      const std::uint32_t* __restrict__ words,
+                                                      const std::int32_t* __restrict__ widths,
+                                                      const double* __restrict__ lscale,
+                                                      const double* __restrict__ rscale,
+                                                      float* __restrict__ out) {
+    const FactorTask t = tasks[blockIdx.x];
+    const double* scale = t.side ? rscale : lscale;
+    std::uint32_t o = 0;
+    for (std::uint32_t j = t.j0; j < t.j0 + t.jn; ++j) {
+        const std::uint32_t bits = std::uint32_t(widths[t.comp_base + j]);
+        const double sc = scale[t.comp_base + j];
+        for (std::uint32_t i = threadIdx.x; i < t.count; i += blockDim.x) {
+            const float v = float(double(decode_side_code(words + t.word, i, o, bits, t.side != 0)) * sc);
+            const std::uint64_t row = t.first + i;
+            out[t.offset + (t.side ? std::uint64_t(j) * t.rows + row : row * t.rank + j)] = v;
+        }
+        o += bits;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 
@@ -1692,26 +1715,89 @@ Plan build_plan(const qh_store_desc& d) {
     return plan;
 }
 
+void upload_rec_tables(const DeviceStore& s);
+
 void reconstruct(const DeviceStore& s, double* out, std::int64_t ld, void* stream) {
     require(out != nullptr, "null output pointer");
     require(ld >= std::int64_t(s.order), "leading dimension is smaller than the matrix order");
     DeviceScope scope(s.device);
-    {
-        std::lock_guard<std::mutex> guard(s.rec.lock);
-        if (!s.rec.uploaded) {
-            std::uint64_t bytes = 0;
-            s.rec.tasks = upload(s.rec.host.tasks, bytes);
-            s.rec.chunks = upload(s.rec.host.chunks, bytes);
-            s.rec.lscale = upload(s.rec.host.lscale, bytes);
-            s.rec.rscale = upload(s.rec.host.rscale, bytes);
-            s.rec.uploaded = true;
-        }
-    }
+    upload_rec_tables(s);
     if (s.rec.host.tasks.empty()) return;
     hipLaunchKernelGGL(reconstruct_tiles, dim3(unsigned(s.rec.host.tasks.size())), dim3(256), 0,
                        static_cast<hipStream_t>(stream), s.rec.tasks, s.rec.chunks, s.words,
                        s.widths, s.rec.lscale, s.rec.rscale, s.perm, out, ld);
     hip_check(hipGetLastError(), "reconstruct launch");
+}
+
+// The reconstruction tables (per-component left and right scales among
+// them) on the device, uploaded on first use.
+void upload_rec_tables(const DeviceStore& s) {
+    std::lock_guard<std::mutex> guard(s.rec.lock);
+    if (s.rec.uploaded) return;
+    std::uint64_t bytes = 0;
+    s.rec.tasks = upload(s.rec.host.tasks, bytes);
+    s.rec.chunks = upload(s.rec.host.chunks, bytes);
+    s.rec.lscale = upload(s.rec.host.lscale, bytes);
+    s.rec.rscale = upload(s.rec.host.rscale, bytes);
+    s.rec.uploaded = true;
+}
+
+FactorPlan* create_factor_plan(const DeviceStore& s, const std::uint64_t* blocks, const int* sides,
+                               const std::uint64_t* offsets, std::uint64_t count,
+                               std::uint64_t out_floats) {
+    require(count == 0 || (blocks && sides && offsets), "null factor slot list");
+    std::vector<FactorTask> tasks;
+    for (std::uint64_t i = 0; i < count; ++i) {
+        const std::uint64_t block = blocks[i];
+        const int side = sides[i];
+        require(block < s.block_to_lowrank.size(), "block index out of range");
+        require(side == 0 || side == 1, "side must be 0 (left) or 1 (right)");
+        const std::int64_t li = s.block_to_lowrank[block];
+        require(li >= 0, "factor slot of a dense or rank-0 block");
+        const auto& chunks = side == 0 ? s.left_chunks[li] : s.right_chunks[li];
+        std::uint32_t rows = 0;
+        for (const auto& c : chunks) rows = std::max(rows, c.first + c.count);
+        const std::uint32_t rank = s.ranks[li];
+        require(offsets[i] <= out_floats && std::uint64_t(rows) * rank <= out_floats - offsets[i],
+                "factor slot past the end of the output");
+        for (const auto& c : chunks)
+            tasks.push_back(FactorTask{c.word, offsets[i], c.first, c.count, c.j0, c.jn,
+                                       std::uint32_t(s.comp_base[li]), std::uint32_t(side), rows,
+                                       rank});
+    }
+    require(tasks.size() < (std::uint64_t{1} << 31), "too many factor chunks");
+    DeviceScope scope(s.device);
+    upload_rec_tables(s);
+    auto* p = new FactorPlan;
+    p->store = &s;
+    p->count = std::uint32_t(tasks.size());
+    try {
+        std::uint64_t bytes = 0;
+        p->tasks = upload(tasks, bytes);
+    } catch (...) {
+        delete p;
+        throw;
+    }
+    return p;
+}
+
+void factor_plan_decode(const FactorPlan& p, float* out, void* stream) {
+    require(out != nullptr, "null output pointer");
+    if (p.count == 0) return;
+    const DeviceStore& s = *p.store;
+    DeviceScope scope(s.device);
+    hipLaunchKernelGGL(decode_factors, dim3(p.count), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       p.tasks, s.words, s.widths, s.rec.lscale, s.rec.rscale, out);
+    hip_check(hipGetLastError(), "factor decode launch");
+}
+
+void destroy_factor_plan(FactorPlan* p) {
+    if (!p) return;
+    if (p->tasks) {
+        DeviceScope scope(p->store->device);
+        (void)hipFree(p->tasks);
+    }
+    delete p;
 }
 
 DeviceStore* create_device_store(const qh_store_desc& d, int device) {

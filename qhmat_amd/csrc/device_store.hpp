@@ -381,6 +381,28 @@ void decode_codes(const DeviceStore& s, std::uint64_t block, int side, std::int3
 // Dense n x n FP64 matrix in source order into device memory (row-major,
 // leading dimension ld), stream-ordered.
 void reconstruct(const DeviceStore& s, double* out, std::int64_t ld, void* stream);
+
+// A list of factor sides to decode into FP32 (code x its side's scale) in
+// one launch, for batched-GEMM applications of the store: side 0 of a block
+// as rows x rank row-major, side 1 as rank x cols row-major (V^T), each at
+// its slot's offset (in floats) of the output.
+struct FactorTask {
+    std::uint64_t word;    // first word of the chunk's side block
+    std::uint64_t offset;  // output offset of the slot (floats)
+    std::uint32_t first, count, j0, jn;  // rows [first, first+count), components [j0, j0+jn)
+    std::uint32_t comp_base, side, rows, rank;
+};
+static_assert(sizeof(FactorTask) == 48, "FactorTask is 48 bytes");
+struct FactorPlan {
+    const DeviceStore* store = nullptr;
+    FactorTask* tasks = nullptr;  // device
+    std::uint32_t count = 0;
+};
+FactorPlan* create_factor_plan(const DeviceStore& s, const std::uint64_t* blocks, const int* sides,
+                               const std::uint64_t* offsets, std::uint64_t count,
+                               std::uint64_t out_floats);
+void factor_plan_decode(const FactorPlan& p, float* out, void* stream);
+void destroy_factor_plan(FactorPlan* p);
 int device_count();
 void profile_begin(DeviceStore& s, std::uint64_t max_applies);
 void set_rhs_block(DeviceStore& s, std::int64_t k, int kb);

@@ -5,18 +5,23 @@ on one MI355X: y = A X for one resident store, per method:
   dense_fp64, dense_fp32        torch.mm (rocBLAS / hipBLASLt GEMM) on the
                                 reconstructed matrix in source order
                                 (the reference's ``dense_fp32``)
-  predecoded_batched_fp32       per exact (m, n, rank) shape bucket, two
-                                batched GEMMs (torch.bmm) on the decoded FP32
-                                factors, diagonal blocks batched the same way,
-                                one permutation gather and one scatter (the
-                                reference's ``adaptive_predecoded_batched``:
-                                decoding done once, outside the timing)
+  h_fp32_batched                per exact (m, n, rank) shape bucket, two
+                                batched GEMMs (torch.bmm) on the unquantized
+                                FP32 balanced factors kept at construction,
+                                diagonal blocks batched the same way, one
+                                permutation gather and one scatter
+  unpacked_batched_fp32         the same GEMMs, every factor decoded from the
+                                resident codes first, in one launch per
+                                application (qh_factor_plan_decode; the
+                                reference's ``adaptive_unpacked_batched``)
+  predecoded_batched_fp32       the same GEMMs on factors decoded once, outside
+                                the timing (``adaptive_predecoded_batched``)
   packed_fused_fp32split,       this library's apply on the packed codes,
   packed_fused_fp64             decoding every time (``adaptive_packed_fused``)
 
-The reference's unquantized-factor methods (``h_fp32_*``) have no
-counterpart: a store holds only the packed factors.  Each result reports its
-relative error against the FP64 fused apply.
+``h_fp32_fused`` (the reference's tiled kernels on unquantized factors) has
+no counterpart here: this library's fused kernels read packed codes only.
+Each result reports its relative error against the FP64 fused apply.
 """
 
 from __future__ import annotations
@@ -31,7 +36,14 @@ from . import device as _device
 
 
 class Harness:
-    def __init__(self, native, device: int = 0):
+    """The methods of one store, set up once (outside the timings): the dense
+    matrix in FP64 and FP32, the exact-shape buckets, a factor-decode plan
+    and the decoded factors; with `factors` (block index -> balanced FP64
+    factors (u sqrt(sigma), v sqrt(sigma)) kept at construction,
+    workloads.build_kernel_hmatrix(keep_factors=True)), the unquantized FP32
+    factors as well."""
+
+    def __init__(self, native, device: int = 0, factors: dict | None = None):
         import torch
 
         self.torch = torch
@@ -44,47 +56,89 @@ class Harness:
         self.dense64 = native.reconstruct_device()
         self.dense32 = self.dense64.float()
         clustered = self.dense64[perm][:, perm]
-        buckets = collections.defaultdict(lambda: {"u": [], "vt": [], "rows": [], "cols": []})
+        groups = collections.defaultdict(lambda: {"blocks": [], "rows": [], "cols": [], "dense": []})
         for index, info in enumerate(native.blocks):
             r = info.rectangle
             m, c = r.row1 - r.row0, r.col1 - r.col0
-            b = buckets[(m, c, 0 if info.dense else info.rank)]
-            b["rows"].append(r.row0)
-            b["cols"].append(r.col0)
+            if not info.dense and info.rank == 0:
+                continue  # contributes nothing
+            g = groups[(m, c, 0 if info.dense else info.rank)]
+            g["blocks"].append(index)
+            g["rows"].append(r.row0)
+            g["cols"].append(r.col0)
             if info.dense:
-                b["u"].append(clustered[r.row0:r.row1, r.col0:r.col1].float())
-            elif info.rank > 0:
-                # decoded factor sides (rank x rows): codes from the device,
-                # times each component's scale
-                fac = [torch.from_numpy(native.decode_codes(index, side).astype(np.float64))
-                       * torch.from_numpy(native.block_words(index, side)[1])[:, None]
-                       for side in (0, 1)]
-                b["u"].append(fac[0].T.contiguous().float().to(dev))   # m x r
-                b["vt"].append(fac[1].contiguous().float().to(dev))    # r x n
+                g["dense"].append(clustered[r.row0:r.row1, r.col0:r.col1].float())
         del clustered
-        self.buckets = []
-        for (m, c, rank), b in buckets.items():
-            if not b["u"]:
-                continue  # rank-0 blocks contribute nothing
-            rows = torch.tensor(b["rows"], device=dev)[:, None] + torch.arange(m, device=dev)
-            cols = torch.tensor(b["cols"], device=dev)[:, None] + torch.arange(c, device=dev)
-            u = torch.stack(b["u"])
-            vt = torch.stack(b["vt"]) if b["vt"] else None
-            self.buckets.append((u, vt, rows.reshape(-1), cols))
+        # factor layout: per low-rank bucket, count x m x r left factors then
+        # count x r x c transposed right factors, in one FP32 buffer
+        blocks, sides, offsets = [], [], []
+        self.buckets = []  # (rows index, cols index, dense stack | None, (off_u, off_vt, shape))
+        total = 0
+        for (m, c, rank), g in groups.items():
+            count = len(g["blocks"])
+            rows = (torch.tensor(g["rows"], device=dev)[:, None] + torch.arange(m, device=dev)).reshape(-1)
+            cols = torch.tensor(g["cols"], device=dev)[:, None] + torch.arange(c, device=dev)
+            if rank == 0:
+                self.buckets.append((rows, cols, torch.stack(g["dense"]), None))
+                continue
+            off_u, off_vt = total, total + count * m * rank
+            for i, b in enumerate(g["blocks"]):
+                blocks += [b, b]
+                sides += [0, 1]
+                offsets += [off_u + i * m * rank, off_vt + i * rank * c]
+            total = off_vt + count * rank * c
+            self.buckets.append((rows, cols, None, (off_u, off_vt, count, m, c, rank)))
+        self.factor_floats = total
+        self.plan = _native.FactorPlan(native, blocks, sides, offsets, max(total, 1), device)
+        self.live = torch.empty(max(total, 1), dtype=torch.float32, device=dev)
+        self.plan.decode(self.live)
+        self.predecoded_factors = self.live.clone()
+        self.fp32_factors = None
+        if factors is not None:
+            host = np.zeros(max(total, 1), dtype=np.float32)
+            for b, side, off in zip(blocks, sides, offsets):
+                u, v = factors[b]
+                f = np.asarray(u if side == 0 else v, dtype=np.float32)
+                a = f if side == 0 else f.T
+                host[off:off + a.size] = a.reshape(-1)
+            self.fp32_factors = torch.from_numpy(host).to(dev)
+        torch.cuda.synchronize()
 
-    def batched(self, x32):
-        """predecoded_batched_fp32: x32 (n, k) FP32 in source order."""
+    def _apply(self, x32, buf):
+        """y = A x over the buckets, low-rank factors from `buf`: x32 (n, k)
+        FP32 in source order; one gather, two batched GEMMs per low-rank
+        bucket (one per dense bucket), one scatter."""
         torch = self.torch
         xc = x32[self.perm]
         yc = torch.zeros_like(xc)
         k = x32.shape[1]
-        for u, vt, rows, cols in self.buckets:
+        for rows, cols, dense, lay in self.buckets:
             xb = xc[cols]                                   # (count, n_b, k)
-            yb = torch.bmm(u, torch.bmm(vt, xb)) if vt is not None else torch.bmm(u, xb)
+            if dense is not None:
+                yb = torch.bmm(dense, xb)
+            else:
+                off_u, off_vt, count, m, c, r = lay
+                u = buf[off_u:off_u + count * m * r].view(count, m, r)
+                vt = buf[off_vt:off_vt + count * r * c].view(count, r, c)
+                yb = torch.bmm(u, torch.bmm(vt, xb))
             yc.index_add_(0, rows, yb.reshape(-1, k))
         y = torch.empty_like(yc)
         y[self.perm] = yc
         return y
+
+    def batched(self, x32):
+        """predecoded_batched_fp32: the factors decoded once, at set-up."""
+        return self._apply(x32, self.predecoded_factors)
+
+    def unpacked(self, x32):
+        """unpacked_batched_fp32: every factor decoded from the resident
+        codes in one launch per application, then the batched GEMMs."""
+        self.plan.decode(self.live)
+        return self._apply(x32, self.live)
+
+    def fp32_batched(self, x32):
+        """h_fp32_batched: the unquantized FP32 factors, no decoding."""
+        return self._apply(x32, self.fp32_factors)
 
     def run(self, k: int, repeats: int = 20, warmup: int = 3, seed: int = 1) -> dict:
         torch = self.torch
@@ -112,9 +166,12 @@ class Harness:
             "dense_fp64": lambda: torch.mm(self.dense64, x64),
             "dense_fp32": lambda: torch.mm(self.dense32, x32),
             "predecoded_batched_fp32": lambda: self.batched(x32),
+            "unpacked_batched_fp32": lambda: self.unpacked(x32),
             "packed_fused_fp32split": lambda: fused(_native.PRECISION_FP32SPLIT),
             "packed_fused_fp64": lambda: fused(_native.PRECISION_FP64),
         }
+        if self.fp32_factors is not None:
+            methods["h_fp32_batched"] = lambda: self.fp32_batched(x32)
         ref = _device.apply(self.native, x64, precision="fp64").clone()
         out = {}
         for name, fn in methods.items():

@@ -14,6 +14,9 @@ keeps the fastest per (store, k, precision) and records it in the store
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+import json
+import os
 from dataclasses import dataclass, field
 
 from . import _native as N
@@ -28,6 +31,63 @@ class TuneResult:
     precision: str
     best: int
     us_per_apply: dict = field(default_factory=dict)  # candidate -> microseconds
+    cached: bool = False  # taken from a TuneCache without timing
+
+
+def store_fingerprint(native) -> str:
+    """What a launch-shape choice depends on, hashed: the order, every
+    block's rectangle, kind and rank and the store's bit totals (the codes'
+    values do not change the timing, their shapes do)."""
+    h = hashlib.sha1()
+    h.update(f"{native.order}:{native.logical_bits}:{native.allocated_bytes}".encode())
+    for b in native.blocks:
+        r = b.rectangle
+        h.update(f"{r.row0},{r.row1},{r.col0},{r.col1},{int(b.dense)},{b.rank};".encode())
+    return h.hexdigest()
+
+
+_BUILD = None
+
+
+def library_build() -> str:
+    """The native library's version and a hash of its file: a rebuilt
+    library (kernels changed) invalidates cached shapes."""
+    global _BUILD
+    if _BUILD is None:
+        with open(N.library_path(), "rb") as f:
+            _BUILD = f"{N.version()}-{hashlib.sha1(f.read()).hexdigest()[:12]}"
+    return _BUILD
+
+
+class TuneCache:
+    """Tuned launch shapes kept across processes in a JSON file, keyed by
+    store fingerprint, device name, library version, precision and k."""
+
+    def __init__(self, path: str | os.PathLike):
+        self.path = os.fspath(path)
+        self.entries: dict = {}
+        if os.path.exists(self.path):
+            with open(self.path) as f:
+                self.entries = json.load(f)
+
+    @staticmethod
+    def key(fingerprint: str, device_name: str, precision: str, k: int) -> str:
+        return f"{fingerprint}|{device_name}|{library_build()}|{precision}|{int(k)}"
+
+    def lookup(self, key: str):
+        e = self.entries.get(key)
+        return None if e is None else int(e["best"])
+
+    def record(self, key: str, result: "TuneResult") -> None:
+        self.entries[key] = {"best": int(result.best),
+                             "us_per_apply": {str(a): round(b, 3) for a, b in
+                                              result.us_per_apply.items()}}
+
+    def save(self) -> None:
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.entries, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
 
 
 def candidates(k: int) -> list[int]:
@@ -42,10 +102,21 @@ def candidates(k: int) -> list[int]:
 
 
 def tune_rhs_block(native, k: int, *, precision: str = "fp32split", device: int = 0,
-                   repeats: int = 20, warmup: int = 3) -> TuneResult:
+                   repeats: int = 20, warmup: int = 3, cache: TuneCache | None = None) -> TuneResult:
     """Times every candidate block for applies of k columns and keeps the
-    fastest in the store.  Needs a HIP device."""
+    fastest in the store.  With a cache, a shape already tuned for this
+    store, device, library version, precision and k is applied without
+    timing, and a new one is recorded and saved.  Needs a HIP device."""
     import torch
+
+    key = None
+    if cache is not None:
+        key = TuneCache.key(store_fingerprint(native), torch.cuda.get_device_name(device),
+                            precision, k)
+        best = cache.lookup(key)
+        if best is not None:
+            native.set_rhs_block(k, best, device)
+            return TuneResult(k=k, precision=precision, best=best, cached=True)
 
     n = native.order
     gen = torch.Generator(device="cpu").manual_seed(k)
@@ -73,11 +144,14 @@ def tune_rhs_block(native, k: int, *, precision: str = "fp32split", device: int 
         result.us_per_apply[kb] = start.elapsed_time(end) * 1e3 / repeats
     result.best = min(result.us_per_apply, key=result.us_per_apply.get)
     native.set_rhs_block(k, result.best, device)
+    if cache is not None:
+        cache.record(key, result)
+        cache.save()
     return result
 
 
 def tune_batch(natives, ks, *, precision: str = "fp32split", device: int = 0,
-               repeats: int = 10) -> list[TuneResult]:
+               repeats: int = 10, cache: TuneCache | None = None) -> list[TuneResult]:
     """Tunes every distinct (store, k) of a batch before the batch is built,
     so that the batch groups its members by the tuned blocks."""
     done = {}
@@ -85,5 +159,5 @@ def tune_batch(natives, ks, *, precision: str = "fp32split", device: int = 0,
         key = (id(native), int(k))
         if key not in done:
             done[key] = tune_rhs_block(native, int(k), precision=precision, device=device,
-                                       repeats=repeats)
+                                       repeats=repeats, cache=cache)
     return list(done.values())

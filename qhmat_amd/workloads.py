@@ -28,8 +28,8 @@ import numpy as np
 
 from . import blockwork as B
 from ._native import NativeHMatrix, Rectangle, spatial_partition
-from .blockwork import (SingleLayer, distances, exponential, inverse_shifted, log_kernel,
-                        range_spectrum)
+from .blockwork import (RectangleSingleLayer, SingleLayer, distances, exponential,
+                        inverse_distance, inverse_shifted, log_kernel, range_spectrum)
 from .matrix import HMatrix, allocate_widths
 
 FULL_MENU = (4, 8, 12, 16, 24, 32)
@@ -79,7 +79,8 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                          support_upper=None, workers: int | None = None,
                          progress: Callable[[int, int], None] | None = None,
                          stress_width: int | None = None, svd: str = "full",
-                         processes: bool | None = None) -> HMatrix:
+                         processes: bool | None = None, kernel_data: np.ndarray | None = None,
+                         keep_factors: bool = False) -> HMatrix:
     """Blockwise, parallel equivalent of from_dense(kernel(points, points), points, ...).
 
     ``progress(done, total)`` is called after every twentieth of the blocks.
@@ -97,7 +98,15 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
     SVD the reference constructor takes: same tolerances and checks, ranks
     equal or one-off, and each block's measured residual still bounds its
     truncation; for the large synthetic stores whose full SVDs would take many
-    CPU-minutes."""
+    CPU-minutes.
+
+    ``kernel_data``: per-point rows handed to the kernel instead of the points
+    (e.g. panel centres, half-widths and normals), in the points' order; the
+    partition still uses the points and support boxes.
+
+    ``keep_factors``: the result also carries ``factors``, block index ->
+    (u sqrt(sigma), v sqrt(sigma)) in FP32, the balanced factors before
+    quantization (the application harness's unquantized methods)."""
     if svd not in ("full", "range"):
         raise ValueError("svd must be 'full' or 'range'")
     p = np.ascontiguousarray(points, dtype=np.float64)
@@ -105,7 +114,10 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
     hi = p if support_upper is None else np.asarray(support_upper, dtype=np.float64)
     part = spatial_partition(p, lo, hi, leaf_size, admissibility)
     perm = np.asarray(part.permutation, dtype=np.int64)
-    clustered = p[perm]
+    data = p if kernel_data is None else np.ascontiguousarray(kernel_data, dtype=np.float64)
+    if data.shape[0] != p.shape[0]:
+        raise ValueError("kernel_data must have one row per point")
+    clustered = data[perm]
     widths = tuple(bit_widths)
     n = p.shape[0]
 
@@ -114,6 +126,7 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                   arithmetic_reserve=arithmetic_reserve, stress=stress_width is not None,
                   svd=svd)
     native = NativeHMatrix(n, part.permutation)
+    kept = {} if keep_factors else None
     workers = workers or min(16, os.cpu_count() or 1)
     blocks = list(part.blocks)
     if processes is None:
@@ -156,12 +169,18 @@ def build_kernel_hmatrix(points: np.ndarray, kernel: Callable, *, tolerance: flo
                     bits = allocate_widths(sigma, widths, quant_budget, (u.shape[0], v.shape[0]),
                                            _box(rect))
                 native.add_low_rank(rect, u, sigma, v, bits, len(widths), residual)
+                if kept is not None and sigma.size:
+                    root = np.sqrt(sigma)
+                    kept[done - 1] = ((u * root).astype(np.float32), (v * root).astype(np.float32))
             nxt = next(it, None)
             if nxt is not None:
                 pending.append((nxt, (pool.submit(_box(nxt)) if processes else
                                    pool.submit(B.block_job, _box(nxt), job))))
     native.finalize()
-    return HMatrix(native, source_array_bytes=n * n * 8)
+    out = HMatrix(native, source_array_bytes=n * n * 8)
+    if kept is not None:
+        out.factors = kept
+    return out
 
 
 def _box(rect: Rectangle) -> tuple:
@@ -286,3 +305,62 @@ def serving_workload(count: int = 16, n_min: int = 2048, n_max: int = 32768,
         if progress is not None:
             progress(f"store {i + 1}/{count}: {name}" + (" (stress)" if stress else ""))
     return natives, shapes
+
+
+# -- the reference application suite's two cases (experiments: geometric_matrices.py) --
+def box_panels(subdivisions: int, extent=(1.0, 1.0, 1.0)):
+    """Axis-aligned square panels covering a closed box: 6 subdivisions^2
+    panels, subdivisions x subdivisions per face; returns the panel rows
+    (centre, half-widths, normal axis) as kernel data."""
+    ext = np.asarray(extent, dtype=np.float64)
+    rows = []
+    for axis in range(3):
+        a, b = [d for d in range(3) if d != axis]
+        ga = (np.arange(subdivisions) + 0.5) * ext[a] / subdivisions - ext[a] / 2
+        gb = (np.arange(subdivisions) + 0.5) * ext[b] / subdivisions - ext[b] / 2
+        ua, ub = np.meshgrid(ga, gb, indexing="ij")
+        for side in (-1.0, 1.0):
+            r = np.zeros((subdivisions * subdivisions, 7))
+            r[:, axis] = side * ext[axis] / 2
+            r[:, a], r[:, b] = ua.ravel(), ub.ravel()
+            r[:, 3 + a] = ext[a] / (2 * subdivisions)
+            r[:, 3 + b] = ext[b] / (2 * subdivisions)
+            r[:, 6] = axis
+            rows.append(r)
+    return np.vstack(rows)
+
+
+@dataclass
+class SuiteCase:
+    """A case of the reference's application suite, rebuilt with this
+    library's constructor (leaf 32, admissibility 1, widths 4..32)."""
+    name: str
+    points: np.ndarray
+    kernel: Callable
+    support_lower: np.ndarray | None = None
+    support_upper: np.ndarray | None = None
+    kernel_data: np.ndarray | None = None
+
+    def build(self, tolerance: float, keep_factors: bool = True) -> HMatrix:
+        return build_kernel_hmatrix(self.points, self.kernel, tolerance=tolerance, leaf_size=32,
+                                    support_lower=self.support_lower,
+                                    support_upper=self.support_upper,
+                                    kernel_data=self.kernel_data, keep_factors=keep_factors,
+                                    processes=False)
+
+
+def suite_inverse_distance_2d(n: int = 2048, seed: int = 7) -> SuiteCase:
+    """kernel_inverse_distance_2d_uniform_n2048: 1/r on n points uniform in
+    [-0.5, 0.5]^2, zero diagonal."""
+    pts = np.random.default_rng(seed).uniform(-0.5, 0.5, size=(n, 2))
+    return SuiteCase(f"inverse_distance_2d_uniform_n{n}", pts, inverse_distance)
+
+
+def suite_cube_bem(subdivisions: int = 16) -> SuiteCase:
+    """bem3d_inverse_cube_n1536: constant-panel collocation of the 1/(4 pi r)
+    single layer on the unit cube's surface, 6 x 16^2 panels, each panel's
+    integral in closed form (self terms included), support boxes = panels."""
+    data = box_panels(subdivisions)
+    centres, half = data[:, :3], data[:, 3:6]
+    return SuiteCase(f"cube_bem_n{data.shape[0]}", centres, RectangleSingleLayer(),
+                     support_lower=centres - half, support_upper=centres + half, kernel_data=data)

@@ -54,3 +54,14 @@ def test_tuner_keeps_a_candidate_and_batches_follow(torch):
     torch.cuda.synchronize()
     for view, native, x in zip(batch.views(out), [a, b], xs):
         np.testing.assert_array_equal(view.cpu().numpy(), native.matmul(x, True, precision="fp32split"))
+
+
+def test_tuner_cache_skips_timing(torch, tmp_path):
+    native = H.ragged_square(n=700, leaf=24).native
+    cache = T.TuneCache(tmp_path / "tune.json")
+    first = T.tune_rhs_block(native, 6, repeats=2, cache=cache)
+    assert not first.cached and first.best in T.candidates(6)
+    native.set_rhs_block(6, 0)
+    again = T.tune_rhs_block(native, 6, repeats=2, cache=T.TuneCache(tmp_path / "tune.json"))
+    assert again.cached and again.best == first.best and again.us_per_apply == {}
+    native.set_rhs_block(6, 0)

@@ -101,3 +101,70 @@ def test_worker_error_raises_promptly():
         W.build_kernel_hmatrix(pts, W.exponential, tolerance=1e-8, leaf_size=64,
                                processes=True, workers=3)
     assert time.time() - t < 25
+
+
+def _panel_quadrature(x, centre, half, axis, n=300):
+    """Midpoint rule for (1/4 pi) * integral of 1/|x - y| over a panel."""
+    a, b = [d for d in range(3) if d != axis]
+    us = centre[a] + half[a] * ((np.arange(n) + 0.5) / n * 2 - 1)
+    vs = centre[b] + half[b] * ((np.arange(n) + 0.5) / n * 2 - 1)
+    uu, vv = np.meshgrid(us, vs, indexing="ij")
+    y = np.zeros(uu.shape + (3,))
+    y[..., a], y[..., b], y[..., axis] = uu, vv, centre[axis]
+    r = np.linalg.norm(y - x, axis=-1)
+    return (1 / r).mean() * 4 * half[a] * half[b] / (4 * np.pi)
+
+
+def test_panel_integral_closed_form():
+    """The closed-form panel integral against quadrature away from the panel,
+    and the textbook self term: integral of 1/r over a square of side s about
+    its centre = 4 s ln(1 + sqrt 2)."""
+    rng = np.random.default_rng(0)
+    k = B.RectangleSingleLayer()
+    for axis in range(3):
+        centre = rng.uniform(-1, 1, 3)
+        half = np.full(3, 0.1)
+        half[axis] = 0.0
+        panel = np.concatenate([centre, half, [axis]])[None]
+        for x in (rng.uniform(-1, 1, 3), centre + np.array([0.3, -0.2, 0.25])):
+            got = k(np.concatenate([x, np.zeros(4)])[None], panel)[0, 0]
+            assert got == pytest.approx(_panel_quadrature(x, centre, half, axis), rel=1e-6)
+        got = k(np.concatenate([centre, np.zeros(4)])[None], panel)[0, 0]
+        assert got == pytest.approx(4 * 0.2 * np.log(1 + np.sqrt(2)) / (4 * np.pi), rel=1e-14)
+
+
+def test_suite_cases():
+    cube = W.suite_cube_bem(4)
+    assert cube.kernel_data.shape == (96, 7) and cube.points.shape == (96, 3)
+    # every panel lies on the unit cube's surface, half-width 1/8 in-plane
+    assert np.allclose(np.abs(cube.points).max(axis=1), 0.5)
+    h = cube.build(1e-2)
+    x = np.random.default_rng(2).normal(size=(96, 2))
+    a = cube.kernel(cube.kernel_data, cube.kernel_data)
+    assert np.all(np.abs(h @ x - a @ x).max(axis=0) <= h.reconstruction_bound * np.abs(x).sum(axis=0))
+    assert set(h.factors) == {i for i, b in enumerate(h.native.blocks) if not b.dense and b.rank}
+    two = W.suite_inverse_distance_2d(300)
+    d = two.kernel(two.points, two.points)
+    assert np.all(np.diag(d) == 0) and np.all(d[~np.eye(300, dtype=bool)] > 0)
+
+
+def test_tune_cache_round_trip(tmp_path):
+    """The tuner's persisted cache: fingerprints tell stores apart and are
+    stable, entries survive a save and a reload (timing itself needs a GPU,
+    tests/test_gpu_tune.py)."""
+    from qhmat_amd import tune as T
+
+    a = W.build_kernel_hmatrix(W.uniform_points(600, 2, seed=1), W.exponential,
+                               tolerance=1e-6, leaf_size=32).native
+    b = W.build_kernel_hmatrix(W.uniform_points(600, 2, seed=2), W.exponential,
+                               tolerance=1e-6, leaf_size=32).native
+    fa = T.store_fingerprint(a)
+    assert fa == T.store_fingerprint(a) and fa != T.store_fingerprint(b)
+    cache = T.TuneCache(tmp_path / "tune.json")
+    key = T.TuneCache.key(fa, "dev", "fp32split", 7)
+    assert cache.lookup(key) is None
+    cache.record(key, T.TuneResult(k=7, precision="fp32split", best=4, us_per_apply={4: 1.5, 16: 2.0}))
+    cache.save()
+    again = T.TuneCache(tmp_path / "tune.json")
+    assert again.lookup(key) == 4
+    assert again.lookup(T.TuneCache.key(fa, "dev", "fp64", 7)) is None
