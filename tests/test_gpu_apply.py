@@ -218,3 +218,19 @@ def test_concurrent_host_threads(stores):
     stop.set()
     tuner.join(timeout=60)
     assert not errors, errors[0]
+
+
+def test_side_stream_and_strided_input(stores, torch):
+    """apply on a caller's side stream with a non-contiguous x: the input copy,
+    the output and the workspace are ordered on that stream, and the result
+    equals the default-stream one."""
+    native, orc = stores["ragged_square"]
+    x_full = torch.from_numpy(np.random.default_rng(7).normal(size=(native.order, 10))).cuda()
+    x = x_full[:, ::2]  # non-contiguous, 5 columns
+    want = native.apply_device(x.contiguous())
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    got = native.apply_device(x, stream=side)
+    side.synchronize()
+    assert torch.equal(got, want)
+    H.same_result(got.cpu().numpy(), orc.matmul(x.cpu().numpy()), "fp64")

@@ -82,3 +82,27 @@ def test_batch_rejects_bad_input(members):
         DeviceBatch(natives[:2], [1])
     with pytest.raises(ValueError, match="right-hand-side"):
         DeviceBatch(natives[:1], [0])
+
+
+def test_store_release_refused_while_batched(members):
+    """A store's device copy cannot be released while a batch (or its graph)
+    holds its device pointers; it can once they are gone."""
+    import gc
+
+    import torch
+
+    from qhmat_amd.batch import DeviceBatch
+
+    natives, ks = members
+    native = H.ragged_square(n=500, leaf=20, seed=21).native
+    batch = DeviceBatch([native, natives[1]], [2, 3])
+    graph = batch.graph(batch.empty(), batch.empty())
+    with pytest.raises(ValueError):
+        native.release_device(0)
+    del graph, batch
+    gc.collect()
+    torch.cuda.synchronize()
+    native.release_device(0)
+    # and it comes back on the next use
+    x = np.random.default_rng(2).normal(size=(native.order, 2))
+    H.same_result(native.matmul(x, True), H.oracle_of(native).matmul(x), "fp64")

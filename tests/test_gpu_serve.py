@@ -105,3 +105,39 @@ def test_nonfinite_requests_raise(natives):
         with pytest.raises(ValueError, match="nonfinite"):
             fut.result(timeout=60)
         assert ok.result(timeout=60).shape == (n, 1)
+
+
+def test_oversized_request_rejected_and_lone_request_fits(natives):
+    """A request wider than the server's max_rhs is refused at submit; one of
+    exactly max_rhs columns is served alone."""
+    import torch
+
+    from qhmat_amd.serve import Server
+
+    native = natives["line"]
+    with Server(hbm_budget=1 << 32, max_rhs=8) as server:
+        server.admit("line", native)
+        with pytest.raises(ValueError, match="max_rhs"):
+            server.submit("line", torch.ones(native.order, 9, dtype=torch.float64, device="cuda"))
+        x = torch.randn(native.order, 8, dtype=torch.float64, device="cuda")
+        got = server.submit("line", x).result(timeout=60)
+        H.same_result(got.cpu().numpy(), H.oracle_of(native).matmul(x.cpu().numpy()), "fp64")
+
+
+def test_evict_fails_queued_requests(natives):
+    """Requests still queued for a store that is evicted fail with KeyError
+    (the server is not started here, so nothing takes them first)."""
+    import torch
+
+    from qhmat_amd.serve import Server
+
+    native = natives["user"]
+    server = Server(hbm_budget=1 << 32)
+    server.admit("user", native)
+    futs = [server.submit("user", torch.ones(native.order, 2, dtype=torch.float64, device="cuda"))
+            for _ in range(3)]
+    server.evict("user")
+    for f in futs:
+        with pytest.raises(KeyError):
+            f.result(timeout=10)
+    assert server.resident_bytes == 0
