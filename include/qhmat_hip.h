@@ -237,8 +237,8 @@ qh_status qh_decode_codes(const qh_store *store, uint64_t block, int side, int32
  * at `offsets[i]` floats into an output of `out_floats` floats.  Each value
  * is float(code x that side's component scale).  qh_factor_plan_decode
  * decodes every listed side from the resident words in one stream-ordered
- * launch; the plan holds a task table in device memory until destroyed
- * (destroy it before its store). */
+ * launch; the plan holds a task table in device memory until destroyed,
+ * and the store refuses destruction (QH_EINVAL) while a plan on it exists. */
 typedef struct qh_factor_plan qh_factor_plan;
 qh_status qh_factor_plan_create(const qh_store *store, const uint64_t *blocks, const int32_t *sides,
                                 const uint64_t *offsets, uint64_t count, uint64_t out_floats,

@@ -346,8 +346,14 @@ class WorkerPool:
                        for _ in range(max(1, workers))]
         self._next = 0
         self._unread = [0] * len(self._procs)  # answers not yet received, per worker
-        for p in self._procs:
-            _send(p.stdin, job)
+        try:
+            for p in self._procs:
+                _send(p.stdin, job)
+        except BaseException:
+            for p in self._procs:
+                p.kill()
+                p.wait()
+            raise
 
     def submit(self, rect):
         i = self._next

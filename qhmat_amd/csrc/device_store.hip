@@ -1778,6 +1778,7 @@ FactorPlan* create_factor_plan(const DeviceStore& s, const std::uint64_t* blocks
         delete p;
         throw;
     }
+    s.pins.fetch_add(1);  // the store refuses destruction while the plan exists
     return p;
 }
 
@@ -1797,6 +1798,7 @@ void destroy_factor_plan(FactorPlan* p) {
         DeviceScope scope(p->store->device);
         (void)hipFree(p->tasks);
     }
+    p->store->pins.fetch_sub(1);
     delete p;
 }
 

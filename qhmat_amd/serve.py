@@ -385,7 +385,15 @@ def closed_loop_load(server: Server, names: list[str], orders: dict[str, int], r
                         finished.set()
                 return
             left[0] -= 1
-            server.submit(name, xs[(name, k)]).add_done_callback(step)
+            try:
+                fut = server.submit(name, xs[(name, k)])
+            except Exception as e:  # e.g. the store was evicted: the session ends
+                with lock:
+                    state["errors"].append(e)
+                left[0] = 0
+                step()
+                return
+            fut.add_done_callback(step)
 
         step()
 

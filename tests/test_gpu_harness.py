@@ -89,3 +89,18 @@ def test_factor_plan_rejects_bad_slots(built, torch):
         N.FactorPlan(native, [low], [2], [0], 1 << 20)
     with pytest.raises(ValueError):
         N.FactorPlan(native, [low], [0], [0], 1)  # output too small
+
+
+def test_store_release_refused_while_a_factor_plan_exists(torch):
+    import gc
+
+    from qhmat_amd import _native as N
+
+    native = H.ragged_square(n=400, leaf=20, seed=4).native
+    low = next(i for i, b in enumerate(native.blocks) if not b.dense and b.rank > 0)
+    plan = N.FactorPlan(native, [low], [0], [0], 1 << 16)
+    with pytest.raises(ValueError):
+        native.release_device(0)
+    del plan
+    gc.collect()
+    native.release_device(0)
