@@ -62,6 +62,26 @@ __device__ __forceinline__ int decode_side_code(const std::uint32_t* side, std::
 }
 
 
+// Workgroups of a 1-D launch are handed to the eight XCDs in turn (workgroup
+// L runs on XCD L mod 8), and each XCD has its own L2.  A launch of `groups`
+// work groups with `slices` slices each (right-hand-side slices, row groups)
+// takes group g on XCD g mod 8 and runs its slices there one after another,
+// so the slices, which read the same codes, share one L2, while the groups
+// stay spread over the XCDs in turn (batches mix groups of very different
+// cost, so a contiguous share per XCD would not balance).  The grid has
+// xcd_grid(groups, slices) workgroups; workgroup L takes group *g and slice
+// *s, with *g >= groups for the padding.
+constexpr std::uint32_t kXcds = 8;
+__host__ __device__ __forceinline__ void xcd_order(std::uint32_t L, std::uint32_t slices,
+                                                   std::uint32_t& g, std::uint32_t& s) {
+    const std::uint32_t j = L / kXcds;
+    g = kXcds * (j / slices) + L % kXcds;
+    s = j % slices;
+}
+inline std::uint32_t xcd_grid(std::uint64_t groups, std::uint64_t slices) {
+    return std::uint32_t((groups + kXcds - 1) / kXcds * kXcds * slices);
+}
+
 // LDS written by some lanes of a wave and read by others: keep the compiler
 // from moving accesses across this point and wait for the LDS queue.
 __device__ __forceinline__ void wave_lds_sync() {

@@ -2148,6 +2148,9 @@ void emulate_wide(const Plan& pl, const double* x, std::int64_t k, double* y, bo
       for (int wv = 0; wv < kWideWaves; ++wv) {
         const WideRun& wr = pl.wruns_l[g * kWideWaves + wv];
         check(wr.tile == run.tile && wr.set == run.set, "a pass-L workgroup spans two runs");
+        // the kernel walks a workgroup's wave runs as one range
+        check(wv == 0 || wr.begin == pl.wruns_l[g * kWideWaves + wv - 1].end,
+              "the wave runs of a pass-L workgroup are not one range");
         check(wr.begin <= wr.end && wr.end <= pl.witems.size(), "pass L run range");
         for (std::uint32_t i = wr.begin; i < wr.end; ++i) {
             const ItemDev& it = pl.witems[i];

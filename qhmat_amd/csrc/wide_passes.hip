@@ -318,30 +318,65 @@ __device__ __forceinline__ void right_group(const WideView& V, std::uint32_t g, 
 }
 
 
-// Pass L.  A workgroup owns one run of a row tile's items; its kWideWaves
-// waves each take a share of the run (their own wave runs) and form all 64
-// rows of the tile from it: per K-step a lane decodes the codes of four rows
-// (one per 16-row block) and the B operands are shared by the four blocks.
-// The waves' FP64 sums are added in a fixed order at the end.
+// Pass L.  A workgroup owns one run of a row tile's items and one row group
+// of the tile: RB = 4 / NS of its four 16-row blocks, for KC = 16 NS
+// right-hand sides.  Its kWideWaves waves each take a share of the run
+// (their own wave runs) and form the group's rows for all KC right-hand
+// sides: per K-step a lane decodes the codes of its RB rows (one per row
+// block) and each decoded code is the A operand of NS MFMAs, one per
+// 16-column slice, while each B operand serves the RB row blocks.  The
+// waves' FP64 sums are added in a fixed order at the end.  A wave stages
+// only its group's rows of an item's side block (the words of lanes
+// 16 rb0 .. 16 (rb0 + RB) - 1 of every word row), so the row groups of one
+// run read disjoint bytes.  The row groups of a run are neighbouring
+// workgroups of one XCD (xcd_order), so the partial sums they all read stay
+// in its L2.
 template <int KC, bool FP32> struct LeftShape {
-    static constexpr int NS = KC / 16;
-    static constexpr int kVrow = kLdsHead + kStageBytes;  // per wave: head, stage, rows
+    static constexpr int NS = KC / 16;     // 16-column slices per wave
+    static constexpr int RB = 4 / NS;      // 16-row blocks per wave
+    static constexpr int NG = NS;          // row groups (workgroups per run)
+    static constexpr int W = 16 * RB;      // words of a staged word row
+    static constexpr int kQuads = RB;      // 16-byte loads per lane per item (16 word rows)
+    // per wave: a head row, 16 word rows, a slack row, then the virtual rows
+    static constexpr int kHead = W * 4;
+    static constexpr int kVrow = kHead + 17 * W * 4;
     static constexpr int kWaveBytes = kVrow + 128 * 4;
     // the final sums, one 16-wide slice at a time: [wave][row block][t][lane]
-    static constexpr int kSumBytes = kWideWaves * 4 * 4 * 64 * 8;
+    static constexpr int kSumBytes = kWideWaves * RB * 4 * 64 * 8;
     static constexpr int kBytes = kWideWaves * kWaveBytes > kSumBytes ? kWideWaves * kWaveBytes
                                                                        : kSumBytes;
+    static_assert(kWaveBytes % 16 == 0, "LDS alignment");
 };
 
-// What a pass-L wave fetches of an item one item ahead: the item's left side
-// block (four 16-byte groups per lane), the component widths (lanes =
-// components) and the B operands of its first four K-steps.  A diagonal or
-// empty item fetches nothing (every load out of range).
+// What a pass-L wave fetches of an item one item ahead: its row group's
+// words of the item's left side block (RB 16-byte groups per lane), the
+// component widths (lanes = components) and the B operands of its first
+// four K-steps.  A diagonal or empty item fetches nothing (every load out of
+// range).
 template <int KC, bool FP32> struct LeftFetch {
-    quad_t q[kItemQuads / 64];
+    quad_t q[LeftShape<KC, FP32>::kQuads];
     std::uint32_t bits;
     real_t<FP32> b[4][KC / 16];
 };
+
+// Item record `slot` of a group of 64 held one per lane (r0, r1: its two
+// 16-byte halves), as wave-uniform fields, sizes clamped to the stage.
+__device__ __forceinline__ ItemDev item_from_lanes(const quad_t& r0, const quad_t& r1,
+                                                   std::uint32_t slot) {
+    ItemDev f;
+    f.quad = __builtin_amdgcn_readlane(r0.x, slot);
+    const std::uint32_t d1 = __builtin_amdgcn_readlane(r0.y, slot);
+    f.quads = std::uint16_t(min(d1 & 0xFFFFu, std::uint32_t(kItemQuads)));
+    f.kind = std::uint8_t((d1 >> 16) & 0xFFu);
+    f.count = std::uint8_t(min(d1 >> 24, std::uint32_t(kItemComponents)));
+    f.comp0 = __builtin_amdgcn_readlane(r0.z, slot);
+    f.col = __builtin_amdgcn_readlane(r0.w, slot);
+    f.cols = 0;
+    f.zrow = __builtin_amdgcn_readlane(r1.y, slot);
+    f.zstride = __builtin_amdgcn_readlane(r1.z, slot);
+    f.nseg = 0;
+    return f;
+}
 
 // B operands of K-steps 4 c .. 4 c + 3 of a kLeft item: lane (q, r) holds,
 // for step s and slice n, the partial sum of virtual row 4 (4 c + s) + q
@@ -374,29 +409,46 @@ __device__ __forceinline__ void left_b(const WideView& V, const ItemDev& f, std:
     }
 }
 
+// The lane's 16-byte group t of its row group's words: word row w, words
+// 4 cq .. 4 cq + 3 of the group's W words (lanes 16 rb0 + 4 cq ..).
+template <int KC, bool FP32>
+__device__ __forceinline__ void left_group_word(int lane, int t, std::uint32_t& w, std::uint32_t& cq) {
+    using S = LeftShape<KC, FP32>;
+    const std::uint32_t idx = std::uint32_t(lane + 64 * t);
+    w = idx / std::uint32_t(S::W / 4);
+    cq = idx % std::uint32_t(S::W / 4);
+}
+
 template <int KC, bool FP32>
 __device__ __forceinline__ void left_fetch(const WideView& V, const ItemDev& f, int lane, int p0,
-                                           LeftFetch<KC, FP32>& F) {
+                                           int rb0, LeftFetch<KC, FP32>& F) {
+    using S = LeftShape<KC, FP32>;
     const bool u = f.kind == kLeft && f.count > 0;
     const __amdgpu_buffer_rsrc_t data =
         buffer(V.words + 4ull * f.quad, u ? std::uint32_t(f.quads) * 16u : 0u);
 #pragma unroll
-    for (int t = 0; t < kItemQuads / 64; ++t)
-        F.q[t] = __builtin_amdgcn_raw_buffer_load_b128(data, std::uint32_t(lane + 64 * t) * 16u, 0, 0);
+    for (int t = 0; t < S::kQuads; ++t) {
+        std::uint32_t w, cq;
+        left_group_word<KC, FP32>(lane, t, w, cq);
+        const std::uint32_t word = 64u * w + 16u * std::uint32_t(rb0) + 4u * cq;
+        F.q[t] = __builtin_amdgcn_raw_buffer_load_b128(data, word * 4u, 0, 0);
+    }
     F.bits = __builtin_amdgcn_raw_buffer_load_b32(buffer(V.widths + f.comp0, u ? f.count * 4u : 0u),
                                                   std::uint32_t(lane) * 4u, 0, 0);
     left_b<KC, FP32>(V, f, 0, lane >> 4, lane & 15, p0, F.b);
 }
 
-// One fetched item of a pass-L wave run, added to the wave's sums of all 64
-// rows (sums[rb][n] slot t: row 16 rb + 4 q + t, right-hand side p0 + 16 n + r).
+// One fetched item of a pass-L wave run, added to the wave's sums of its
+// row group (sums[i][n] slot t: row 16 (rb0 + i) + 4 q + t, right-hand side
+// p0 + 16 n + r).
 template <int KC, bool FP32>
 __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
                                           const LeftFetch<KC, FP32>& F, std::uint32_t R, int lane,
-                                          int p0, std::uint8_t* wl, f64x4 (&sums)[4][KC / 16]) {
+                                          int p0, int rb0, std::uint8_t* wl,
+                                          f64x4 (&sums)[LeftShape<KC, FP32>::RB][KC / 16]) {
     using Real = real_t<FP32>;
     using S = LeftShape<KC, FP32>;
-    constexpr int NS = KC / 16;
+    constexpr int NS = KC / 16, RB = S::RB;
     const int q = lane >> 4, r = lane & 15;
     const int k = V.k;
     if (f.count == 0) return;  // past the end of the run
@@ -404,8 +456,7 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
         // R x cnt FP64 values, column-major; lane (q, r) supplies row
         // 16 rb + a_row(r) of column 4 s + q and x of that column (buffer
         // loads past the item's values and columns read zero; rows past the
-        // tile read its last row and are never written; x at a valid address,
-        // the columns past k never written)
+        // tile read its last row and are never written)
         const std::uint32_t cnt = min(std::uint32_t(f.count), std::uint32_t(kDenseColumns));
         const __amdgpu_buffer_rsrc_t dv = buffer(V.words + 4ull * f.quad, cnt * R * 8u);
         const __amdgpu_buffer_rsrc_t pc = buffer(V.perm + f.col, cnt * 4u);
@@ -419,38 +470,44 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
             for (int n = 0; n < NS; ++n) b[s][n] = V.x[xrow + min(p0 + 16 * n + r, k - 1)];
         }
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const std::uint32_t row = min(16u * rb + std::uint32_t(a_row<false>(r)), R - 1u);
+        for (int i = 0; i < RB; ++i) {
+            const std::uint32_t row =
+                min(16u * std::uint32_t(rb0 + i) + std::uint32_t(a_row<false>(r)), R - 1u);
 #pragma unroll
             for (int s = 0; s < kDenseColumns / 4; ++s) {
                 const std::uint32_t c = 4u * std::uint32_t(s) + std::uint32_t(q);
                 const double a = load_f64(dv, (c * R + row) * 8u);
 #pragma unroll
-                for (int n = 0; n < NS; ++n) sums[rb][n] = Mma<false>::mad(a, b[s][n], sums[rb][n]);
+                for (int n = 0; n < NS; ++n) sums[i][n] = Mma<false>::mad(a, b[s][n], sums[i][n]);
             }
         }
         return;
     }
-    quad_t* stage_q = reinterpret_cast<quad_t*>(wl + kLdsHead);
-    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wl + kLdsHead);
+    // the group's words: word row w of the block at stage[W w .. W w + W - 1]
+    quad_t* stage_q = reinterpret_cast<quad_t*>(wl + S::kHead);
+    const std::uint32_t* stage = reinterpret_cast<const std::uint32_t*>(wl + S::kHead);
     std::uint32_t* vrow = reinterpret_cast<std::uint32_t*>(wl + S::kVrow);
+    const std::uint32_t rows16 = std::uint32_t(f.quads) / 16u;  // word rows of the block
 #pragma unroll
-    for (int t = 0; t < kItemQuads / 64; ++t)
-        if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
+    for (int t = 0; t < S::kQuads; ++t) {
+        std::uint32_t w, cq;
+        left_group_word<KC, FP32>(lane, t, w, cq);
+        if (w < rows16) stage_q[w * std::uint32_t(S::W / 4) + cq] = F.q[t];
+    }
     wave_lds_sync();
     const std::uint32_t nv = build_vrows<FP32>(F.bits, f.count, lane, vrow);
     wave_lds_sync();
     const std::uint32_t arow = FP32 ? std::uint32_t(r) : std::uint32_t(a_row<false>(r));
-    typename Mma<FP32>::acc acc[4][NS];
+    typename Mma<FP32>::acc acc[RB][NS];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int n = 0; n < NS; ++n) acc[rb][n] = Mma<FP32>::zero();
-    // A: the codes of rows 16 rb + arow (lanes past the tile's rows read the
-    // zero words of the block's padding lanes); a high part carries its 2^16
-    // (exact); rows past the item are zero
-    // K-steps from row 2 W on are whole components (no split handling), and
-    // only the last one can reach past the rows
+        for (int n = 0; n < NS; ++n) acc[i][n] = Mma<FP32>::zero();
+    // A: the codes of rows 16 (rb0 + i) + arow (lanes past the tile's rows
+    // read the zero words of the block's padding lanes); a high part carries
+    // its 2^16 (exact); rows past the item are zero.  K-steps from row 2 W
+    // on are whole components (no split handling), and only the last one
+    // can reach past the rows.
     const std::uint32_t split_rows = 2u * (nv - std::uint32_t(f.count));
     auto run_chunk = [&](std::uint32_t c, const Real (&b)[4][NS]) {
 #pragma unroll
@@ -460,12 +517,12 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
             const std::uint32_t m = vrow_at(vrow, v, nv);
             const std::uint32_t E = vrow_end(m);
             const std::uint32_t kk = E >> 5, sh = 32u - vrow_bits(m), part = vrow_part(m);
-            const std::uint32_t* hi = stage + 64 * kk + arow;
-            const std::uint32_t* lo = stage + 64 * (int(kk) - 1) + int(arow);
+            const std::uint32_t* hi = stage + std::uint32_t(S::W) * kk + arow;
+            const std::uint32_t* lo = stage + (int(S::W) * (int(kk) - 1) + int(arow));
             const bool split = v0 < split_rows, edge = v0 + 4u > nv;
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                int code = static_cast<int>(__builtin_amdgcn_alignbit(hi[16 * rb], lo[16 * rb], E)) >> sh;
+            for (int i = 0; i < RB; ++i) {
+                int code = static_cast<int>(__builtin_amdgcn_alignbit(hi[16 * i], lo[16 * i], E)) >> sh;
                 Real a;
                 if (split) {
                     code = code_part(code, part);
@@ -475,7 +532,7 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
                 }
                 if (edge) a = v < nv ? a : Real(0);
 #pragma unroll
-                for (int n = 0; n < NS; ++n) acc[rb][n] = Mma<FP32>::mad(a, b[s][n], acc[rb][n]);
+                for (int n = 0; n < NS; ++n) acc[i][n] = Mma<FP32>::mad(a, b[s][n], acc[i][n]);
             }
         }
     };
@@ -497,22 +554,23 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
         run_chunk(0, F.b);
     }
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int n = 0; n < NS; ++n)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) sums[rb][n][t] += double(acc[rb][n][t]);
+            for (int t = 0; t < 4; ++t) sums[i][n][t] += double(acc[i][n][t]);
     wave_lds_sync();  // the next item overwrites the stage and the rows
 }
 
 template <int KC, bool FP32>
-__device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, int p0,
-                                           std::uint8_t* lds) {
+__device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, std::uint32_t rg,
+                                           int p0, std::uint8_t* lds) {
     using S = LeftShape<KC, FP32>;
-    constexpr int NS = S::NS;
+    constexpr int NS = S::NS, RB = S::RB;
     const int tid = int(threadIdx.x);
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63, q = lane >> 4, r = lane & 15;
+    const int rb0 = int(rg) * RB;
     const WideRun run = V.lruns[g * kWideWaves + wave];
     const std::uint32_t begin = __builtin_amdgcn_readfirstlane(run.begin);
     const std::uint32_t end = __builtin_amdgcn_readfirstlane(run.end);
@@ -522,48 +580,58 @@ __device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, i
     const int k = V.k;
     std::uint8_t* wl = lds + wave * S::kWaveBytes;
 
-    f64x4 sums[4][NS];
+    f64x4 sums[RB][NS];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int n = 0; n < NS; ++n) sums[rb][n] = Mma<false>::zero();
+        for (int n = 0; n < NS; ++n) sums[i][n] = Mma<false>::zero();
 
-    // Two fetch buffers alternate, so no register of a load in flight is
-    // copied; both halves of the loop always run (an odd count ends with
-    // an empty item).
-    LeftFetch<KC, FP32> FA, FB;
-    ItemDev fa = item_or_empty(V, begin, end);
-    left_fetch<KC, FP32>(V, fa, lane, p0, FA);
-    for (std::uint32_t i = begin; i < end; i += 2) {
-        const ItemDev fb = item_or_empty(V, i + 1, end);
-        left_fetch<KC, FP32>(V, fb, lane, p0, FB);
-        left_item<KC, FP32>(V, fa, FA, R, lane, p0, wl, sums);
-        fa = item_or_empty(V, i + 2, end);
-        left_fetch<KC, FP32>(V, fa, lane, p0, FA);
-        left_item<KC, FP32>(V, fb, FB, R, lane, p0, wl, sums);
+    // The run's item records 64 at a time, one per lane, so that an item's
+    // fetch does not wait for its record.  Two fetch buffers alternate, so
+    // no register of a load in flight is copied; both halves of the loop
+    // always run (an odd count ends with an empty item).
+    for (std::uint32_t g0 = begin; g0 < end; g0 += 64) {
+        const std::uint32_t n = min(64u, end - g0);
+        const __amdgpu_buffer_rsrc_t recs = buffer(V.items, kOutOfRange);
+        const std::uint32_t roff = std::uint32_t(lane) < n ? (g0 + std::uint32_t(lane)) * 32u : kOutOfRange;
+        const quad_t r0 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff, 0, 0);
+        const quad_t r1 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff + 16u, 0, 0);
+        auto slot = [&](std::uint32_t i) { return i < n ? item_from_lanes(r0, r1, i) : ItemDev{}; };
+        LeftFetch<KC, FP32> FA, FB;
+        ItemDev fa = slot(0);
+        left_fetch<KC, FP32>(V, fa, lane, p0, rb0, FA);
+        for (std::uint32_t i = 0; i < n; i += 2) {
+            const ItemDev fb = slot(i + 1);
+            left_fetch<KC, FP32>(V, fb, lane, p0, rb0, FB);
+            left_item<KC, FP32>(V, fa, FA, R, lane, p0, rb0, wl, sums);
+            fa = slot(i + 2);
+            left_fetch<KC, FP32>(V, fa, lane, p0, rb0, FA);
+            left_item<KC, FP32>(V, fb, FB, R, lane, p0, rb0, wl, sums);
+        }
     }
 
     // The four waves' sums, one slice at a time, through LDS: wave w adds
-    // row block w of every wave as (s0 + s1) + (s2 + s3) and writes rows
-    // 16 w + 4 q + t of right-hand side p0 + 16 n + r.
+    // slot t = w of the group's row blocks of every wave as (s0 + s1) +
+    // (s2 + s3) and writes rows 16 (rb0 + i) + 4 q + w of right-hand side
+    // p0 + 16 n + r.
     double* part = reinterpret_cast<double*>(lds);
     double* sets = reinterpret_cast<double*>(static_cast<std::uint8_t*>(V.zw) +
                                              V.z_rows * std::uint64_t(k) * sizeof(double));
     for (int n = 0; n < NS; ++n) {
         __syncthreads();  // the stage (the first slice) or the previous slice is done with
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) part[((wave * 4 + rb) * 4 + t) * 64 + lane] = sums[rb][n][t];
+            for (int t = 0; t < 4; ++t) part[((wave * RB + i) * 4 + t) * 64 + lane] = sums[i][n][t];
         __syncthreads();
         const int p = p0 + 16 * n + r;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int i = 0; i < RB; ++i) {
             double v[kWideWaves];
 #pragma unroll
-            for (int w = 0; w < kWideWaves; ++w) v[w] = part[((w * 4 + wave) * 4 + t) * 64 + lane];
+            for (int w = 0; w < kWideWaves; ++w) v[w] = part[((w * RB + i) * 4 + wave) * 64 + lane];
             const double sum = (v[0] + v[1]) + (v[2] + v[3]);
-            const std::uint32_t row = 16u * std::uint32_t(wave) + 4u * std::uint32_t(q) + std::uint32_t(t);
+            const std::uint32_t row = 16u * std::uint32_t(rb0 + i) + 4u * std::uint32_t(q) + std::uint32_t(wave);
             if (row < R && p < k) {
                 if (run.set == kNoSet)
                     V.y[std::int64_t(V.perm[row0 + row]) * V.ldy + p] = sum;
@@ -643,22 +711,38 @@ inline unsigned sum_chunks(int k) { return unsigned((kTile * std::uint32_t(k) + 
 // Waves per SIMD the register allocation must allow: pass R; pass L, whose
 // two fetch buffers grow with the right-hand sides per pass.
 template <bool FP32> constexpr int wide_occupancy() { return FP32 ? 4 : 3; }
-// Pass L runs 16 right-hand sides per launch slice whatever pass R's width:
-// its waves hold FP64 sums of all 64 rows, 32 registers per 16 columns (32
-// per slice spill at three waves per SIMD).
-template <bool FP32> constexpr int left_kc(int) { return 16; }
-template <int LKC, bool FP32> constexpr int left_occupancy() { return FP32 ? 4 : 3; }
+// Pass L takes as many right-hand sides per launch slice as pass R in FP32
+// mode; FP64 mode keeps 16 (a wave's FP64 item sums and B operands for more
+// columns would hold it to two waves per SIMD, which measured slower than
+// decoding once per 16 columns).
+template <bool FP32> constexpr int left_kc(int kc) { return FP32 ? kc : 16; }
+template <int LKC, bool FP32> constexpr int left_occupancy() { return FP32 ? (LKC == 16 ? 4 : 3) : 3; }
+template <int LKC, bool FP32> constexpr int batch_left_occupancy() { return LKC == 16 ? 3 : 2; }
 
+// Pass R and pass L take (workgroup, slice) pairs in XCD order (xcd_order):
+// the slices of one workgroup's work (right-hand-side slices, pass L's row
+// groups) run on one XCD one after another, so the codes they all read are
+// fetched into one L2.
 template <int KC, bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_right(WideView V) {
+__global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void wide_right(
+    WideView V, std::uint32_t groups, std::uint32_t slices) {
     __shared__ __align__(16) std::uint8_t lds[RightShape<KC, FP32>::kBytes];
-    right_group<KC, FP32>(V, blockIdx.x, int(blockIdx.y) * KC, lds);
+    std::uint32_t g, s;
+    xcd_order(blockIdx.x, slices, g, s);
+    if (g >= groups) return;  // the grid's padding to a multiple of 8
+    right_group<KC, FP32>(V, g, int(s) * KC, lds);
 }
 
 template <int LKC, bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, (left_occupancy<LKC, FP32>())) void wide_left(WideView V) {
+__global__ __launch_bounds__(64 * kWideWaves, (left_occupancy<LKC, FP32>())) void wide_left(
+    WideView V, std::uint32_t groups, std::uint32_t slices) {
     __shared__ __align__(16) std::uint8_t lds[LeftShape<LKC, FP32>::kBytes];
-    left_group<LKC, FP32>(V, blockIdx.x, int(blockIdx.y) * LKC, lds);
+    // slice s: row group s mod NG of right-hand-side slice s / NG
+    std::uint32_t g, s;
+    xcd_order(blockIdx.x, slices, g, s);
+    if (g >= groups) return;
+    constexpr std::uint32_t NG = LeftShape<LKC, FP32>::NG;
+    left_group<LKC, FP32>(V, g, s % NG, int(s / NG) * LKC, lds);
 }
 
 __global__ __launch_bounds__(kSumChunk) void wide_sum(WideView V) {
@@ -694,25 +778,32 @@ __device__ __forceinline__ WideView bind(const WideView* __restrict__ members, s
 template <int KC, bool FP32>
 __global__ __launch_bounds__(64 * kWideWaves, wide_occupancy<FP32>()) void batch_wide_right(
     const WideView* __restrict__ members, const std::uint32_t* __restrict__ first,
-    std::uint32_t count, const double* x, double* y, std::uint8_t* zw) {
+    std::uint32_t count, const double* x, double* y, std::uint8_t* zw, std::uint32_t slices) {
     __shared__ __align__(16) std::uint8_t lds[RightShape<KC, FP32>::kBytes];
-    const std::uint32_t m = owner(first, count, blockIdx.x);
+    std::uint32_t g, s;
+    xcd_order(blockIdx.x, slices, g, s);
+    if (g >= first[count]) return;  // the grid's padding to a multiple of 8
+    const std::uint32_t m = owner(first, count, g);
     const WideView V = bind(members, m, x, y, zw);
-    const int p0 = int(blockIdx.y) * KC;
+    const int p0 = int(s) * KC;
     if (p0 >= V.k) return;  // the whole workgroup leaves together
-    right_group<KC, FP32>(V, blockIdx.x - first[m], p0, lds);
+    right_group<KC, FP32>(V, g - first[m], p0, lds);
 }
 
 template <int LKC, bool FP32>
-__global__ __launch_bounds__(64 * kWideWaves, 3) void batch_wide_left(
+__global__ __launch_bounds__(64 * kWideWaves, (batch_left_occupancy<LKC, FP32>())) void batch_wide_left(
     const WideView* __restrict__ members, const std::uint32_t* __restrict__ first,
-    std::uint32_t count, const double* x, double* y, std::uint8_t* zw) {
+    std::uint32_t count, const double* x, double* y, std::uint8_t* zw, std::uint32_t slices) {
     __shared__ __align__(16) std::uint8_t lds[LeftShape<LKC, FP32>::kBytes];
-    const std::uint32_t m = owner(first, count, blockIdx.x);
+    std::uint32_t g, s;
+    xcd_order(blockIdx.x, slices, g, s);
+    if (g >= first[count]) return;
+    const std::uint32_t m = owner(first, count, g);
     const WideView V = bind(members, m, x, y, zw);
-    const int p0 = int(blockIdx.y) * LKC;
+    constexpr std::uint32_t NG = LeftShape<LKC, FP32>::NG;
+    const int p0 = int(s / NG) * LKC;
     if (p0 >= V.k) return;
-    left_group<LKC, FP32>(V, blockIdx.x - first[m], p0, lds);
+    left_group<LKC, FP32>(V, g - first[m], s % NG, p0, lds);
 }
 
 template <bool FP32>
@@ -742,8 +833,8 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
     const unsigned passes = unsigned((V.k + KC - 1) / KC);
     if (ev) hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
     if (c.right_groups) {
-        hipLaunchKernelGGL((wide_right<KC, FP32>), dim3(c.right_groups, passes),
-                           dim3(64 * kWideWaves), 0, st, V);
+        hipLaunchKernelGGL((wide_right<KC, FP32>), dim3(xcd_grid(c.right_groups, passes)),
+                           dim3(64 * kWideWaves), 0, st, V, c.right_groups, passes);
         hip_check(hipGetLastError(), "wide pass R launch");
     }
     if (c.zsums) {
@@ -752,8 +843,9 @@ void launch_wide_kc(const WideView& V, const WideCounts& c, hipStream_t st, hipE
     }
     if (ev) hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
     constexpr int LKC = left_kc<FP32>(KC);
-    hipLaunchKernelGGL((wide_left<LKC, FP32>), dim3(c.left_groups, unsigned((V.k + LKC - 1) / LKC)),
-                       dim3(64 * kWideWaves), 0, st, V);
+    const unsigned lslices = unsigned((V.k + LKC - 1) / LKC) * LeftShape<LKC, FP32>::NG;
+    hipLaunchKernelGGL((wide_left<LKC, FP32>), dim3(xcd_grid(c.left_groups, lslices)),
+                       dim3(64 * kWideWaves), 0, st, V, c.left_groups, lslices);
     hip_check(hipGetLastError(), "wide pass L launch");
     if (c.sums) {
         hipLaunchKernelGGL(wide_sum, dim3(c.sums, sum_chunks(V.k)), dim3(kSumChunk), 0, st, V);
@@ -768,8 +860,9 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
     const unsigned passes = unsigned((c.max_k + KC - 1) / KC);
     const std::uint32_t n = std::uint32_t(c.member_count);
     if (c.right_groups) {
-        hipLaunchKernelGGL((batch_wide_right<KC, FP32>), dim3(c.right_groups, passes),
-                           dim3(64 * kWideWaves), 0, st, c.members, c.first_right, n, x, y, zw);
+        hipLaunchKernelGGL((batch_wide_right<KC, FP32>),
+                           dim3(xcd_grid(c.right_groups, passes)),
+                           dim3(64 * kWideWaves), 0, st, c.members, c.first_right, n, x, y, zw, passes);
         hip_check(hipGetLastError(), "batched wide pass R launch");
     }
     if (c.zsums) {
@@ -778,9 +871,10 @@ void launch_batch_wide_kc(const WideBatchClass& c, const double* x, double* y, s
         hip_check(hipGetLastError(), "batched wide partial-sum reduction launch");
     }
     constexpr int LKC = left_kc<FP32>(KC);
+    const unsigned lslices = unsigned((c.max_k + LKC - 1) / LKC) * LeftShape<LKC, FP32>::NG;
     hipLaunchKernelGGL((batch_wide_left<LKC, FP32>),
-                       dim3(c.left_groups, unsigned((c.max_k + LKC - 1) / LKC)),
-                       dim3(64 * kWideWaves), 0, st, c.members, c.first_left, n, x, y, zw);
+                       dim3(xcd_grid(c.left_groups, lslices)),
+                       dim3(64 * kWideWaves), 0, st, c.members, c.first_left, n, x, y, zw, lslices);
     hip_check(hipGetLastError(), "batched wide pass L launch");
     if (c.sums) {
         hipLaunchKernelGGL(batch_wide_sum, dim3(c.sums, sum_chunks(c.max_k)), dim3(kSumChunk), 0,
