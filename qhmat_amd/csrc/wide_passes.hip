@@ -359,25 +359,6 @@ template <int KC, bool FP32> struct LeftFetch {
     real_t<FP32> b[4][KC / 16];
 };
 
-// Item record `slot` of a group of 64 held one per lane (r0, r1: its two
-// 16-byte halves), as wave-uniform fields, sizes clamped to the stage.
-__device__ __forceinline__ ItemDev item_from_lanes(const quad_t& r0, const quad_t& r1,
-                                                   std::uint32_t slot) {
-    ItemDev f;
-    f.quad = __builtin_amdgcn_readlane(r0.x, slot);
-    const std::uint32_t d1 = __builtin_amdgcn_readlane(r0.y, slot);
-    f.quads = std::uint16_t(min(d1 & 0xFFFFu, std::uint32_t(kItemQuads)));
-    f.kind = std::uint8_t((d1 >> 16) & 0xFFu);
-    f.count = std::uint8_t(min(d1 >> 24, std::uint32_t(kItemComponents)));
-    f.comp0 = __builtin_amdgcn_readlane(r0.z, slot);
-    f.col = __builtin_amdgcn_readlane(r0.w, slot);
-    f.cols = 0;
-    f.zrow = __builtin_amdgcn_readlane(r1.y, slot);
-    f.zstride = __builtin_amdgcn_readlane(r1.z, slot);
-    f.nseg = 0;
-    return f;
-}
-
 // B operands of K-steps 4 c .. 4 c + 3 of a kLeft item: lane (q, r) holds,
 // for step s and slice n, the partial sum of virtual row 4 (4 c + s) + q
 // for right-hand side p0 + 16 n + r.  The item's rows are in virtual-row
@@ -536,19 +517,18 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
             }
         }
     };
-    // chunk 0 came with the fetch; later ones load one chunk ahead (both
-    // halves of the loop always run: an odd count runs one chunk past the
-    // rows, whose A operands are zero)
+    // chunk 0 came with the fetch; chunk 1 is requested before chunk 0 is
+    // computed, later ones (more than 32 virtual rows: rare) where they are
+    // used
     const std::uint32_t nchunks = (nv + 15u) / 16u;
     if (nchunks > 1) {
-        Real b0[4][NS], b1[4][NS];
+        Real b1[4][NS];
         left_b<KC, FP32>(V, f, 1, q, r, p0, b1);
         run_chunk(0, F.b);
-        for (std::uint32_t c = 1; c < nchunks; c += 2) {
-            left_b<KC, FP32>(V, f, c + 1, q, r, p0, b0);
+        run_chunk(1, b1);
+        for (std::uint32_t c = 2; c < nchunks; ++c) {
+            left_b<KC, FP32>(V, f, c, q, r, p0, b1);
             run_chunk(c, b1);
-            left_b<KC, FP32>(V, f, c + 2, q, r, p0, b1);
-            run_chunk(c + 1, b0);
         }
     } else {
         run_chunk(0, F.b);
@@ -586,28 +566,19 @@ __device__ __forceinline__ void left_group(const WideView& V, std::uint32_t g, s
 #pragma unroll
         for (int n = 0; n < NS; ++n) sums[i][n] = Mma<false>::zero();
 
-    // The run's item records 64 at a time, one per lane, so that an item's
-    // fetch does not wait for its record.  Two fetch buffers alternate, so
-    // no register of a load in flight is copied; both halves of the loop
-    // always run (an odd count ends with an empty item).
-    for (std::uint32_t g0 = begin; g0 < end; g0 += 64) {
-        const std::uint32_t n = min(64u, end - g0);
-        const __amdgpu_buffer_rsrc_t recs = buffer(V.items, kOutOfRange);
-        const std::uint32_t roff = std::uint32_t(lane) < n ? (g0 + std::uint32_t(lane)) * 32u : kOutOfRange;
-        const quad_t r0 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff, 0, 0);
-        const quad_t r1 = __builtin_amdgcn_raw_buffer_load_b128(recs, roff + 16u, 0, 0);
-        auto slot = [&](std::uint32_t i) { return i < n ? item_from_lanes(r0, r1, i) : ItemDev{}; };
-        LeftFetch<KC, FP32> FA, FB;
-        ItemDev fa = slot(0);
+    // Two fetch buffers alternate, so no register of a load in flight is
+    // copied; both halves of the loop always run (an odd count ends with
+    // an empty item).
+    LeftFetch<KC, FP32> FA, FB;
+    ItemDev fa = item_or_empty(V, begin, end);
+    left_fetch<KC, FP32>(V, fa, lane, p0, rb0, FA);
+    for (std::uint32_t i = begin; i < end; i += 2) {
+        const ItemDev fb = item_or_empty(V, i + 1, end);
+        left_fetch<KC, FP32>(V, fb, lane, p0, rb0, FB);
+        left_item<KC, FP32>(V, fa, FA, R, lane, p0, rb0, wl, sums);
+        fa = item_or_empty(V, i + 2, end);
         left_fetch<KC, FP32>(V, fa, lane, p0, rb0, FA);
-        for (std::uint32_t i = 0; i < n; i += 2) {
-            const ItemDev fb = slot(i + 1);
-            left_fetch<KC, FP32>(V, fb, lane, p0, rb0, FB);
-            left_item<KC, FP32>(V, fa, FA, R, lane, p0, rb0, wl, sums);
-            fa = slot(i + 2);
-            left_fetch<KC, FP32>(V, fa, lane, p0, rb0, FA);
-            left_item<KC, FP32>(V, fb, FB, R, lane, p0, rb0, wl, sums);
-        }
+        left_item<KC, FP32>(V, fb, FB, R, lane, p0, rb0, wl, sums);
     }
 
     // The four waves' sums, one slice at a time, through LDS: wave w adds

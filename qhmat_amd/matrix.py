@@ -268,7 +268,7 @@ class HMatrix:
 
     # -- apply (matrix.py:321-333) ---------------------------------------------------
     def matmul(self, x, *, device: str = "cpu", precision: str = "fp64"):
-        """Apply to an ``(n,)`` vector or ``(n, p)`` array.
+        """y = H x for x of shape ``(n,)`` or ``(n, p)``, in source order.
 
         ``device='cuda'`` runs on the GPU; a torch device tensor is applied in
         place on the device and a device tensor is returned, anything else is

@@ -415,8 +415,12 @@ def closed_loop_load(server: Server, names: list[str], orders: dict[str, int], r
     if state["errors"]:
         raise state["errors"][0]
     done = server.completed
+    # the Poisson draw of session starts offers len(plans) * session
+    # requests over `duration` seconds: the rate the run actually offered
     out = {"mode": "closed-loop sessions", "offered_rate": rate, "sessions": len(plans),
-           "session_requests": session, "requests": done, "seconds": round(elapsed, 4),
+           "session_requests": session,
+           "offered_rate_drawn": len(plans) * session / duration if duration > 0 else 0.0,
+           "requests": done, "seconds": round(elapsed, 4),
            "throughput_rps": done / elapsed, "passes": server.passes}
     out.update({f"latency_{k}_ms": v for k, v in server.latency_ms().items()})
     return out
