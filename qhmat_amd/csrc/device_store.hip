@@ -426,11 +426,12 @@ struct Staged {
 };
 
 // Moves a fetched item into the wave's LDS: its data, its x rows (lanes =
-// columns; zero past the item's columns) or kLeft partial sums (lanes =
+// columns; zero past the item's columns; in FP32 mode a kFused or kRight
+// item's rows go straight to the FP32 table) or kLeft partial sums (lanes =
 // pairs; zero past them), and its widths and scale products.  Every
 // register of the fetch buffer is dead afterwards, so the next fetch into it
 // needs no copy of a load still in flight.
-template <int KB>
+template <int KB, bool FP32>
 __device__ __forceinline__ void stage_item(const Item& f, const Fetch<KB>& F, int lane,
                                            std::uint8_t* wave_lds) {
     quad_t* stage_q = reinterpret_cast<quad_t*>(wave_lds + kLdsHead);
@@ -438,8 +439,14 @@ __device__ __forceinline__ void stage_item(const Item& f, const Fetch<KB>& F, in
 #pragma unroll
     for (int t = 0; t < kItemQuads / 64; ++t)
         if (std::uint32_t(lane + 64 * t) < f.quads) stage_q[lane + 64 * t] = F.q[t];
+    if (FP32 && (f.kind == kFused || f.kind == kRight)) {
+        float* xf = reinterpret_cast<float*>(wave_lds + wave_lds_xf<KB>());
 #pragma unroll
-    for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+        for (int p = 0; p < KB; ++p) xf[lane * KB + p] = float(F.in[p]);
+    } else {
+#pragma unroll
+        for (int p = 0; p < KB; ++p) xs[lane * KB + p] = F.in[p];
+    }
     reinterpret_cast<double*>(wave_lds + wave_lds_meta<KB>())[lane] = F.scale;
     reinterpret_cast<std::uint32_t*>(wave_lds + wave_lds_meta<KB>() + 512)[lane] = F.bits;
 }
@@ -507,17 +514,12 @@ __device__ __forceinline__ void valu_item(const MemberView& M, const Item& f, co
         std::uint32_t total;
         const std::uint32_t rmeta_l = side_meta(comp, bits_l, lane, total);
         const std::uint32_t* lstage = stage + side_words(total);
-        // x rows in the arithmetic precision (FP32 mode: converted once)
+        // x rows in the arithmetic precision (FP32 mode: converted at staging)
         const Real* xr;
-        if constexpr (FP32) {
-            float* xf = reinterpret_cast<float*>(wave_lds + wave_lds_xf<KB>());
-#pragma unroll
-            for (int p = 0; p < KB; ++p) xf[lane * KB + p] = float(xs[lane * KB + p]);
-            wave_lds_sync();
-            xr = xf;
-        } else {
+        if constexpr (FP32)
+            xr = reinterpret_cast<const float*>(wave_lds + wave_lds_xf<KB>());
+        else
             xr = xs;
-        }
         // In FP32 mode the components up to the last one wider than 24 bits
         // (the widest come first) go in blocks of 4, each deciding the split
         // for itself; then blocks of 16 while more than 8 remain, then one of
@@ -592,7 +594,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
         // decode.
         for (std::uint32_t s = 0; s < n; s += 2) {
             {
-                stage_item<KB>(fa, FA, lane, wave_lds);
+                stage_item<KB, FP32>(fa, FA, lane, wave_lds);
                 fetch<KB>(M, fc, lane, p0, kk, FA);
                 wave_lds_sync();
                 compute_item<KB, FP32, PASS>(M, fa, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
@@ -605,7 +607,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
             // read zero, so that both steps always run and the waits for
             // loads in flight stay exact across the loop.
             {
-                stage_item<KB>(fb, FB, lane, wave_lds);
+                stage_item<KB, FP32>(fb, FB, lane, wave_lds);
                 fetch<KB>(M, fd, lane, p0, kk, FB);
                 wave_lds_sync();
                 compute_item<KB, FP32, PASS>(M, fb, staged_meta<KB>(wave_lds, lane), lane, p0, kk,
