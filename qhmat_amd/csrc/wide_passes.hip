@@ -362,30 +362,35 @@ template <int KC, bool FP32> struct LeftFetch {
 // B operands of K-steps 4 c .. 4 c + 3 of a kLeft item: lane (q, r) holds,
 // for step s and slice n, the partial sum of virtual row 4 (4 c + s) + q
 // for right-hand side p0 + 16 n + r.  The item's rows are in virtual-row
-// order, so the addresses need only the record.  The loads are
-// unconditional: a row past the item reads its last row (its A operand is
-// zero) and a right-hand side past k reads column k - 1 (an MFMA result
-// column depends on its own B column only, and those are never written).
+// order, so the addresses need only the record: one lane offset per chunk,
+// the step as a uniform offset and the slice as an immediate.  The loads are
+// unconditional: a row past the item lies past the buffer's bytes and reads
+// zero (its A operand is zero too), and a right-hand side past k reads the
+// next row's first values or zero (an MFMA result column depends on its own
+// B column only, and those columns are never written).
 template <int KC, bool FP32>
 __device__ __forceinline__ void left_b(const WideView& V, const ItemDev& f, std::uint32_t c,
                                        int q, int r, int p0, real_t<FP32> (&b)[4][KC / 16]) {
     using Real = real_t<FP32>;
-    const int k = V.k;
+    constexpr std::uint32_t esz = sizeof(Real);
+    const std::uint32_t k = std::uint32_t(V.k);
     const bool u = f.kind == kLeft && f.count > 0;
-    const __amdgpu_buffer_rsrc_t zb = buffer(
-        static_cast<const Real*>(V.zw) + std::uint64_t(f.zrow) * std::uint64_t(k),
-        u ? f.zstride * std::uint32_t(k) * std::uint32_t(sizeof(Real)) : 0u);
+    const __amdgpu_buffer_rsrc_t zb = buffer(static_cast<const Real*>(V.zw) + std::uint64_t(f.zrow) * k,
+                                             u ? f.zstride * k * esz : 0u);
+    const std::uint32_t lane_off = ((16u * c + std::uint32_t(q)) * k + std::uint32_t(p0 + r)) * esz;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const std::uint32_t v = min(16u * c + 4u * std::uint32_t(s) + std::uint32_t(q), f.zstride - 1u);
+        const int step_off = int(4u * std::uint32_t(s) * k * esz);
 #pragma unroll
         for (int n = 0; n < KC / 16; ++n) {
-            const std::uint32_t p = std::uint32_t(min(p0 + 16 * n + r, k - 1));
-            const std::uint32_t off = (v * std::uint32_t(k) + p) * std::uint32_t(sizeof(Real));
-            if constexpr (FP32)
-                b[s][n] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(zb, off, 0, 0));
-            else
-                b[s][n] = load_f64(zb, off);
+            const std::uint32_t off = lane_off + 16u * esz * std::uint32_t(n);
+            if constexpr (FP32) {
+                b[s][n] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(zb, off, step_off, 0));
+            } else {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(zb, off, step_off, 0);
+                b[s][n] = __longlong_as_double(static_cast<long long>(
+                    (static_cast<unsigned long long>(v[1]) << 32) | static_cast<unsigned long long>(v[0])));
+            }
         }
     }
 }
