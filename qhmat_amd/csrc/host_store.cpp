@@ -383,6 +383,10 @@ void HostStore::finalize() {
 // Host backend.  Each low-rank block's right factor is reduced against x once
 // (all right-hand sides together), then each row slab is owned by one thread,
 // which adds its blocks in insertion order: no shared writes, fixed order.
+// Every sum and product is the reference's (core.cpp:360-390: decoded value
+// = code x scale, dot = sum over k in order, y += left value x dot per
+// component in order, blocks in insertion order), so the result is bitwise
+// the reference's for any thread count.
 void HostStore::matmul(const double* x, std::uint64_t rhs, double* y, int threads) const {
     require(finalized_, "matrix must be finalized before use");
     require(rhs > 0 && rhs <= std::numeric_limits<std::uint64_t>::max() / order_,
@@ -402,7 +406,7 @@ void HostStore::matmul(const double* x, std::uint64_t rhs, double* y, int thread
         loop();
         for (auto& t : pool) t.join();
     };
-    // z[b][j*rhs + p] = sR_j * sum_k codeR(k, j) x[perm[c0 + k], p]
+    // z[b][j*rhs + p] = sum_k (codeR(k, j) sR_j) x[perm[c0 + k], p], k in order
     std::vector<std::vector<double>> z(blocks_.size());
     run(blocks_.size(), [&](std::size_t bi) {
         const auto& b = blocks_[bi];
@@ -413,11 +417,10 @@ void HostStore::matmul(const double* x, std::uint64_t rhs, double* y, int thread
             const std::uint32_t* w = b.right.words.data() + b.right.offsets[j];
             double* zj = z[bi].data() + j * rhs;
             for (std::uint64_t k = 0; k < b.cols(); ++k) {
-                const double c = double(read_code(w, k, b.widths[j]));
+                const double c = double(read_code(w, k, b.widths[j])) * b.right.scales[j];
                 const double* xr = x + perm_[b.rect.col0 + k] * rhs;
                 for (std::uint64_t p = 0; p < rhs; ++p) zj[p] += c * xr[p];
             }
-            for (std::uint64_t p = 0; p < rhs; ++p) zj[p] *= b.right.scales[j];
         }
     });
     std::vector<std::uint64_t> cuts{0, order_};

@@ -39,7 +39,8 @@ def test_oracle_reproduces_reference_bitwise(name):
 def test_product_loads_reference_file(name, tmp_path):
     path, g = _load(name)
     h = qhmat.HMatrix.load(path)
-    np.testing.assert_allclose(h @ g["x"], g["y"], rtol=1e-12, atol=1e-12)
+    # the host backend follows the reference's loop order: bitwise its y
+    np.testing.assert_array_equal(h @ g["x"], g["y"])
     assert h.reconstruction_bound == float(g["bound"])
     assert h.logical_bits == int(g["logical_bits"])
     if "dense" in g:
