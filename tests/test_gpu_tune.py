@@ -39,6 +39,23 @@ def test_every_block_gives_the_same_bits(torch, precision):
     H.same_result(outs[matrix[0]], outs[vector[0]], precision)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32split"])
+def test_several_launch_slices_give_the_same_bits(torch, precision):
+    # k = 70: two 64-wide launch slices (the second 6 columns wide), three
+    # 32-wide or five 16-wide ones; pass L splits each slice into row groups
+    native = H.ragged_square(n=900, leaf=30).native
+    x = np.random.default_rng(4).normal(size=(native.order, 70))
+    outs = {}
+    for kb in (16, 32, 64):
+        native.set_rhs_block(70, kb)
+        outs[kb] = native.matmul(x, True, precision=precision)
+    native.set_rhs_block(70, 0)
+    for kb in (32, 64):
+        np.testing.assert_array_equal(outs[kb], outs[16])
+    want = H.oracle_of(native).matmul(x)
+    H.same_result(outs[16], want, precision)
+
+
 def test_tuner_keeps_a_candidate_and_batches_follow(torch):
     a = H.ragged_square(n=700, leaf=24).native
     b = H.line_kernel(300, 16, 1e-6, widths=(4, 8, 12, 16, 24, 32))[0].native
