@@ -5,16 +5,17 @@
 // deterministic (fixed summation order, one writer per output element, no
 // floating-point atomics):
 //
+//  permute x in clustered order into the workspace.
 //  pass 1  four workgroups of four waves per row tile, each wave running one
 //          of the tile's 16 item runs: the tile's diagonal (kDense) and
 //          fused (kFused: z and U z inside the item) contributions, summed
 //          in a fixed tree per workgroup into one of four row-sum sets of
 //          the workspace, and a share of the kRight items: partial sums
 //          z[c-tile][j] = sum_{c in tile} codeR(c, j) x[c] into the workspace.
-//  pass 2  the same shape over the tile's kLeft items (U z with z summed
-//          over the block's column tiles), into four more row-sum sets.
-//  combine one small workgroup per row tile: the eight sets in a fixed tree,
-//          written once to y[perm[row]].
+//  pass 2  one workgroup of sixteen waves per row tile over the tile's kLeft
+//          items (U z with z summed over the block's column tiles); it adds
+//          its waves and the four pass-1 sets in a fixed tree and writes
+//          y[perm[row]] once.
 //
 // Every wave walks a contiguous list of items with the data of the next two
 // items (<= 4 KiB each, four 16-byte buffer loads per lane), their component
@@ -112,10 +113,10 @@ template <int KB> struct Fetch {
 };
 
 
-// Row sets per tile in the workspace (four workgroups per pass), and x in
+// Row sets per tile in the workspace (pass 1's four workgroups), and x in
 // clustered order after them: xc[i][p] = x[perm[i]][p], written once per
 // apply (permute_tile) so that an item's x rows are one contiguous run.
-constexpr int kRowSets = 2 * kPass1Groups;
+constexpr int kRowSets = kPass1Groups;
 __device__ __forceinline__ double* clustered_x(const MemberView& M) {
     return M.zp + (M.z_rows + std::uint64_t(kRowSets) * M.order) * std::uint64_t(M.k);
 }
@@ -656,17 +657,17 @@ __device__ __forceinline__ void combine_rows(const double (&total)[KB], const do
     }
 }
 
-// The row sums of a tile: each pass's 16 item runs (each summed in item
-// order from zero) go to four workgroups of four waves; workgroup q of pass
-// P sums its waves as ((r0 + r1) + (r2 + r3)) into its own set of rows in the
-// workspace (set 4(P-1) + q), and the combine launch forms
+// The row sums of a tile: pass 1's 16 item runs (each summed in item order
+// from zero) go to four workgroups of four waves; workgroup q sums its waves
+// as 0 + ((r0 + r1) + (r2 + r3)) into set q of the workspace's rows, and pass
+// 2 (tile_pass2) forms
 //   y = ((g0 + g1) + (g2 + g3)) + ((h0 + h1) + (h2 + h3))
-// over the pass-1 sets g and the pass-2 sets h: one fixed tree over the 32
-// runs, the same for every right-hand-side count and launch shape.
+// over these sets g and its own four groups of waves h: one fixed tree over
+// the 32 runs, the same for every right-hand-side count and launch shape.
 constexpr int kGroupWaves = kWaves / kPass1Groups;
 
-template <int KB, bool FP32, int PASS>
-__device__ __forceinline__ void group_rows(const MemberView& M, std::uint32_t group, int p0,
+template <int KB, bool FP32>
+__device__ __forceinline__ void pass1_rows(const MemberView& M, std::uint32_t group, int p0,
                                            std::uint8_t* lds) {
     constexpr int W = kGroupWaves;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -679,42 +680,18 @@ __device__ __forceinline__ void group_rows(const MemberView& M, std::uint32_t gr
     double total[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) total[p] = 0.0;
-    if (PASS == 1)
-        run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
-                               tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(),
-                               total);
-    else
-        run_items<KB, FP32, 2>(M, 0u, 0u, tile->split2[r], tile->split2[r + 1], p0, kk, R,
-                               lds + wave * wave_lds_bytes<KB>(), total);
+    run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
+                           tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(), total);
     double zero[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zero[p] = 0.0;
-    const std::uint32_t set = (PASS == 1 ? 0u : std::uint32_t(kPass1Groups)) + q;
-    double* rows = M.zp + (M.z_rows + set * M.order + tile->row0 + lane) * std::uint64_t(M.k) + p0;
+    double* rows = M.zp + (M.z_rows + q * M.order + tile->row0 + lane) * std::uint64_t(M.k) + p0;
     combine_rows<KB, W>(total, zero, lds, wave, lane, [&](int pc, double a0, double a1) {
         if (lane < int(R)) {
             if (pc < kk) rows[pc] = a0;
             if (pc + 1 < kk) rows[pc + 1] = a1;
         }
     });
-}
-
-// y rows of tile t: the tree over the eight row sets, one thread per
-// (row, right-hand side) so that reads and writes run along the columns.
-__device__ __forceinline__ void combine_tile(const MemberView& M, std::uint32_t t) {
-    const TileDev* __restrict__ tile = M.tiles + t;
-    const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
-    const std::uint32_t k = std::uint32_t(M.k);
-    const std::uint64_t stride = std::uint64_t(M.order) * k;
-    const double* rows = M.zp + (M.z_rows + tile->row0) * std::uint64_t(k);
-    for (std::uint32_t e = threadIdx.x; e < R * k; e += blockDim.x) {
-        const std::uint32_t i = e / k, p = e - i * k;
-        double v[kRowSets];
-#pragma unroll
-        for (int s = 0; s < kRowSets; ++s) v[s] = rows[std::uint64_t(s) * stride + e];
-        M.y[std::int64_t(M.perm[tile->row0 + i]) * M.ldy + p] =
-            ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-    }
 }
 
 // Pass-1 waves per SIMD the register allocation must allow, by right-hand
@@ -727,17 +704,58 @@ template <int KB, bool FP32>
 __global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_pass1(
     MemberView M) {
     __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
-    group_rows<KB, FP32, 1>(M, blockIdx.x, blockIdx.y * KB, lds);
+    pass1_rows<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
+
+// Pass 2 of tile t as one workgroup of kWaves waves (wave w runs the tile's
+// pass-2 item run w), which then writes y for its rows and right-hand sides
+// [p0, p0 + KB) itself, as the fixed tree over the tile's 32 runs
+//   y = ((g0 + g1) + (g2 + g3)) + ((h0 + h1) + (h2 + h3)),
+// with g the four pass-1 sets from the workspace and h_q = 0 + ((r_4q +
+// r_4q+1) + (r_4q+2 + r_4q+3)) formed from the wave totals r in LDS.  The
+// pass-2 sets never leave the workgroup, and no launch follows.
+template <int KB, bool FP32>
+__device__ __forceinline__ void tile_pass2(const MemberView& M, std::uint32_t t, int p0,
+                                           std::uint8_t* lds) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int kk = min(KB, M.k - p0);
+    const TileDev* __restrict__ tile = M.tiles + t;
+    const std::uint32_t R = min(tile->rows, std::uint32_t(kTile));
+    double total[KB];
+#pragma unroll
+    for (int p = 0; p < KB; ++p) total[p] = 0.0;
+    run_items<KB, FP32, 2>(M, 0u, 0u, tile->split2[wave], tile->split2[wave + 1], p0, kk, R,
+                           lds + wave * wave_lds_bytes<KB>(), total);
+    __syncthreads();  // every wave is done with its stage
+    double* part = reinterpret_cast<double*>(lds);  // [wave][p][row]
+#pragma unroll
+    for (int p = 0; p < KB; ++p) part[(wave * KB + p) * 64 + lane] = total[p];
+    __syncthreads();
+    const std::uint32_t e = threadIdx.x;
+    if (e < R * std::uint32_t(kk)) {
+        const std::uint32_t i = e / std::uint32_t(kk), p = e - i * std::uint32_t(kk);
+        const std::uint64_t k = std::uint64_t(M.k);
+        const double* sets = M.zp + (M.z_rows + tile->row0 + i) * k + std::uint64_t(p0) + p;
+        double g[kPass1Groups], h[kPass1Groups];
+#pragma unroll
+        for (int q = 0; q < kPass1Groups; ++q) {
+            g[q] = sets[std::uint64_t(q) * M.order * k];
+            const double* v = part + (4 * q * KB + int(p)) * 64 + int(i);
+            h[q] = 0.0 + ((v[0] + v[KB * 64]) + (v[2 * KB * 64] + v[3 * KB * 64]));
+        }
+        M.y[std::int64_t(M.perm[tile->row0 + i]) * M.ldy + p0 + std::int64_t(p)] =
+            ((g[0] + g[1]) + (g[2] + g[3])) + ((h[0] + h[1]) + (h[2] + h[3]));
+    }
+}
+static_assert(kWaves == 4 * kPass1Groups, "pass 2 tree: four waves per pass-1 set");
 
 template <int KB, bool FP32>
-__global__ __launch_bounds__(64 * kGroupWaves, pass1_occupancy(KB)) void apply_pass2(
-    MemberView M) {
-    __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
-    group_rows<KB, FP32, 2>(M, blockIdx.x, blockIdx.y * KB, lds);
+__global__ __launch_bounds__(64 * kWaves, 4) void apply_pass2(MemberView M) {
+    __shared__ __align__(16) std::uint8_t lds[kWaves * wave_lds_bytes<KB>()];
+    tile_pass2<KB, FP32>(M, blockIdx.x, blockIdx.y * KB, lds);
 }
 
-__global__ __launch_bounds__(256) void apply_combine(MemberView M) { combine_tile(M, blockIdx.x); }
 
 // x rows of tile t's indices in clustered order (the tiles cover every index
 // once), one thread per (row, right-hand side).
@@ -777,8 +795,8 @@ __device__ __forceinline__ MemberView bind_member(const MemberView* __restrict__
     return M;
 }
 
-template <int KB, bool FP32, int PASS>
-__global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_pass(
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_pass1(
     const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
     std::uint32_t count, const double* x, double* y, double* zp) {
     __shared__ __align__(16) std::uint8_t lds[kGroupWaves * wave_lds_bytes<KB>()];
@@ -787,7 +805,7 @@ __global__ __launch_bounds__(64 * kGroupWaves, batch1_occupancy(KB)) void batch_
     const MemberView M = bind_member(members, m, x, y, zp);
     const int p0 = blockIdx.y * KB;
     if (p0 >= M.k) return;  // the whole workgroup leaves together
-    group_rows<KB, FP32, PASS>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
+    pass1_rows<KB, FP32>(M, blockIdx.x - first[m] * kPass1Groups, p0, lds);
 }
 
 __global__ __launch_bounds__(256) void batch_permute(const MemberView* __restrict__ members,
@@ -799,13 +817,18 @@ __global__ __launch_bounds__(256) void batch_permute(const MemberView* __restric
     permute_tile(M, blockIdx.x - first[m]);
 }
 
-__global__ __launch_bounds__(256) void batch_combine(const MemberView* __restrict__ members,
-                                                    const std::uint32_t* __restrict__ first,
-                                                    std::uint32_t count, const double* x,
-                                                    double* y, double* zp) {
+// Batched pass 2: one workgroup of kWaves waves per (member tile, slice),
+// writing y as tile_pass2 does.
+template <int KB, bool FP32>
+__global__ __launch_bounds__(64 * kWaves, 4) void batch_pass2(
+    const MemberView* __restrict__ members, const std::uint32_t* __restrict__ first,
+    std::uint32_t count, const double* x, double* y, double* zp) {
+    __shared__ __align__(16) std::uint8_t lds[kWaves * wave_lds_bytes<KB>()];
     const std::uint32_t m = locate_member(first, count, blockIdx.x);
     const MemberView M = bind_member(members, m, x, y, zp);
-    combine_tile(M, blockIdx.x - first[m]);
+    const int p0 = blockIdx.y * KB;
+    if (p0 >= M.k) return;  // the whole workgroup leaves together
+    tile_pass2<KB, FP32>(M, blockIdx.x - first[m], p0, lds);
 }
 
 // Test hook: codes of one factor side of one block, from its chunks, written
@@ -1045,11 +1068,9 @@ void launch_apply(const DeviceStore& s, const double* x, std::int64_t ldx, int k
                        dim3(64 * kGroupWaves), 0, stream, M);
     hip_check(hipGetLastError(), "apply pass 1 launch");
     if (ev) hip_check(hipEventRecord(ev[1], stream), "hipEventRecord");
-    hipLaunchKernelGGL((apply_pass2<KB, FP32>), dim3(s.n_tiles * kPass1Groups, passes),
-                       dim3(64 * kGroupWaves), 0, stream, M);
+    hipLaunchKernelGGL((apply_pass2<KB, FP32>), dim3(s.n_tiles, passes), dim3(64 * kWaves), 0,
+                       stream, M);
     hip_check(hipGetLastError(), "apply pass 2 launch");
-    hipLaunchKernelGGL(apply_combine, dim3(s.n_tiles), dim3(256), 0, stream, M);
-    hip_check(hipGetLastError(), "apply combine launch");
     if (ev) hip_check(hipEventRecord(ev[2], stream), "hipEventRecord");
 }
 
@@ -1061,15 +1082,12 @@ void launch_batch_class(const BatchClass& c, const double* x, double* y, double*
     hipLaunchKernelGGL(batch_permute, dim3(c.total_tiles), dim3(256), 0, stream, c.members,
                        c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched permute launch");
-    hipLaunchKernelGGL((batch_pass<KB, FP32, 1>), dim3(c.total_tiles * kPass1Groups, passes),
+    hipLaunchKernelGGL((batch_pass1<KB, FP32>), dim3(c.total_tiles * kPass1Groups, passes),
                        dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 1 launch");
-    hipLaunchKernelGGL((batch_pass<KB, FP32, 2>), dim3(c.total_tiles * kPass1Groups, passes),
-                       dim3(64 * kGroupWaves), 0, stream, c.members, c.first_tile, count, x, y, zp);
+    hipLaunchKernelGGL((batch_pass2<KB, FP32>), dim3(c.total_tiles, passes), dim3(64 * kWaves), 0,
+                       stream, c.members, c.first_tile, count, x, y, zp);
     hip_check(hipGetLastError(), "batched pass 2 launch");
-    hipLaunchKernelGGL(batch_combine, dim3(c.total_tiles), dim3(256), 0, stream, c.members,
-                       c.first_tile, count, x, y, zp);
-    hip_check(hipGetLastError(), "batched combine launch");
 }
 
 template <bool FP32>
