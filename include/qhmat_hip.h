@@ -48,8 +48,8 @@ typedef enum {
 typedef enum {
     /* FP64 throughout: the parity anchor against the FP64 CPU apply. */
     QH_PRECISION_FP64 = 0,
-    /* Codes of <= 24 bits as FP32 (exact); wider codes split into a signed
-     * high half and an unsigned 16-bit low half, each exact in FP32 and
+    /* Codes of <= 24 bits as FP32 (exact); wider codes split into a high
+     * half and a sign-extended low 16-bit half, each exact in FP32 and
      * accumulated apart.  FP32 accumulation inside a block; scales, the
      * diagonal blocks and the cross-block sum into y in FP64. */
     QH_PRECISION_FP32SPLIT = 1
@@ -112,7 +112,8 @@ qh_status qh_device_count(int *count);
 /* Uploads a finalized store to `device` once.  Replaces the per-call flatten +
  * upload at cuda.cu:119-150. */
 qh_status qh_store_create(const qh_store_desc *desc, int device, qh_store **out);
-/* Refused (QH_EINVAL, the store kept) while a batch built on it exists. */
+/* Refused (QH_EINVAL, the store kept) while a batch, a factor plan or a
+ * server holds it. */
 qh_status qh_store_destroy(qh_store *store);
 qh_status qh_store_get_stats(const qh_store *store, qh_store_stats *out);
 
@@ -187,15 +188,77 @@ qh_status qh_graph_destroy(qh_graph *graph);
  * qh_profile_begin arms hipEvents around the launches of the next
  * `max_applies` calls on this store (the events are created here, not in the
  * timed calls); qh_profile_end waits for the recorded events, returns the
- * totals and disarms.  pass_ms[0] = pass 1 (right factors, diagonal and fused
- * blocks), pass_ms[1] = pass 2 (left factors) and the combine launch; the
- * short launch that puts x in clustered order runs before both. */
+ * totals and disarms.  Vector-ALU path: pass_ms[0] = pass 1 (right factors,
+ * diagonal and fused blocks), pass_ms[1] = pass 2 (left factors and the write
+ * of y); the short launch that puts x in clustered order runs before both.
+ * Matrix-core path: pass_ms[0] = pass R and the partial-sum reduction,
+ * pass_ms[1] = pass L and the row-set sum. */
 typedef struct {
     uint64_t applies;
     double pass_ms[2];
 } qh_profile;
 qh_status qh_profile_begin(qh_store *store, uint64_t max_applies);
 qh_status qh_profile_end(qh_store *store, qh_profile *out);
+
+/* ---- serving loop (no counterpart in the reference) ----------------------
+ * Resident stores served from native threads: requests (a store, x of
+ * order x k FP64 row-major in device memory, complete when submitted, and y
+ * of the same shape, written when the request completes) are queued; a
+ * dispatcher thread takes every queued request of the store at the head of
+ * the queue, up to max_rhs columns, as one pass on a free slot (a stream with
+ * its own pass buffers and workspace) -- several requests are gathered into
+ * one apply of all their columns and scattered back -- and a completion
+ * thread marks them done as their passes finish.  A request's result is
+ * bitwise qh_apply's for it alone when both run on the same arithmetic path
+ * (vector ALUs up to 4 columns per pass, matrix cores above), and within the
+ * precision's tolerance otherwise.  Latency = submission to observed
+ * completion. */
+typedef struct qh_server qh_server;
+typedef struct {
+    int32_t device;
+    int32_t precision;     /* qh_precision of every pass                       */
+    int32_t slots;         /* passes in flight at once (streams), 1..16        */
+    int64_t max_rhs;       /* columns per pass and per request, 1..4096        */
+    uint64_t hbm_budget;   /* admitted stores plus pass buffers, bytes         */
+} qh_server_config;
+typedef struct {
+    uint64_t completed, completed_rhs, passes, resident_bytes, buffer_bytes;
+    double p50_ms, p99_ms, mean_ms; /* since the last reset (0 when none)    */
+} qh_server_stats;
+typedef struct {
+    double offered_rate;        /* requests per second asked for             */
+    double offered_rate_drawn;  /* sessions drawn x session length / duration */
+    uint64_t sessions, requests, passes, errors;
+    double seconds, throughput_rps, p50_ms, p99_ms, mean_ms;
+} qh_load_result;
+enum { QH_TICKET_PENDING = 0, QH_TICKET_DONE = 1, QH_TICKET_FAILED = 2 };
+
+qh_status qh_server_create(const qh_server_config *config, qh_server **out);
+/* Stops the threads after the passes in flight; queued requests are dropped. */
+qh_status qh_server_destroy(qh_server *server);
+/* Keeps `store` resident for the server (QH_EINVAL past the HBM budget) and
+ * returns its id; the store refuses destruction until evicted or the server
+ * is destroyed.  Waits for the passes in flight. */
+qh_status qh_server_admit(qh_server *server, const qh_store *store, int32_t *id);
+/* Queued requests of the store fail; passes already launched finish. */
+qh_status qh_server_evict(qh_server *server, int32_t id);
+qh_status qh_server_submit(qh_server *server, int32_t id, const double *x, int64_t k, double *y,
+                           uint64_t *ticket);
+/* Waits up to timeout_s seconds (< 0: no limit); *state is a QH_TICKET_*.
+ * A finished ticket is forgotten once reported. */
+qh_status qh_server_wait(qh_server *server, uint64_t ticket, double timeout_s, int32_t *state);
+qh_status qh_server_get_stats(qh_server *server, qh_server_stats *out);
+qh_status qh_server_reset_stats(qh_server *server);
+/* Closed-loop load (a GMRES client per session): sessions start at Poisson
+ * times, rate / session per second, each on a store of `ids` and a width of
+ * `ks` drawn uniformly, and issue `session` requests one after another, the
+ * next from the completion thread as soon as the previous one is done.  One
+ * request per (store, width) runs first and is dropped from the statistics.
+ * The operands are generated on the device; sessions on one (store, width)
+ * share them (the results are not read).  Returns when every session ends. */
+qh_status qh_server_closed_loop(qh_server *server, const int32_t *ids, uint64_t nids,
+                                const int64_t *ks, uint64_t nks, double rate, double duration,
+                                int32_t session, uint64_t seed, qh_load_result *out);
 
 /* Layout check without a device: builds the device plan of `desc` on the
  * host and runs both passes' index arithmetic against its arrays with every

@@ -15,7 +15,7 @@ from pathlib import Path
 PACKAGE = Path(__file__).resolve().parent
 CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libqhmat_hip.so"
-SOURCES = ["device_store.hip", "wide_passes.hip", "host_store.cpp", "capi.cpp"]
+SOURCES = ["device_store.hip", "wide_passes.hip", "server.hip", "host_store.cpp", "capi.cpp"]
 ARCH = os.environ.get("QHMAT_OFFLOAD_ARCH", "gfx950")
 
 

@@ -11,6 +11,7 @@
 #include "../../include/qhmat_hip.h"
 #include "device_store.hpp"
 #include "host_store.hpp"
+#include "server.hpp"
 
 struct qh_host {
     qhamd::HostStore store;
@@ -34,6 +35,10 @@ struct qh_partition {
 
 struct qh_batch {
     qhamd::Batch* batch = nullptr;
+};
+
+struct qh_server {
+    qhamd::Server* server = nullptr;
 };
 
 struct qh_graph {
@@ -614,6 +619,119 @@ qh_status qh_partition_copy(const qh_partition* p, uint64_t* perm, uint64_t* rec
 qh_status qh_partition_destroy(qh_partition* p) {
     delete p;
     return QH_OK;
+}
+
+// ---- serving loop ---------------------------------------------------------
+
+qh_status qh_server_create(const qh_server_config* config, qh_server** out) {
+    return guarded([&] {
+        need(config, "config");
+        need(out, "output handle");
+        *out = nullptr;
+        qhamd::ServerConfig c;
+        c.device = config->device;
+        c.precision = config->precision;
+        c.slots = config->slots;
+        c.max_rhs = config->max_rhs;
+        c.hbm_budget = config->hbm_budget;
+        auto* s = new qh_server;
+        try {
+            s->server = new qhamd::Server(c);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+qh_status qh_server_destroy(qh_server* server) {
+    return guarded([&] {
+        if (!server) return;
+        delete server->server;
+        delete server;
+    });
+}
+
+qh_status qh_server_admit(qh_server* server, const qh_store* store, int32_t* id) {
+    return guarded([&] {
+        need(server, "server");
+        need(store, "store");
+        need(id, "id");
+        *id = server->server->admit(store->dev);
+    });
+}
+
+qh_status qh_server_evict(qh_server* server, int32_t id) {
+    return guarded([&] {
+        need(server, "server");
+        server->server->evict(id);
+    });
+}
+
+qh_status qh_server_submit(qh_server* server, int32_t id, const double* x, int64_t k, double* y,
+                           uint64_t* ticket) {
+    return guarded([&] {
+        need(server, "server");
+        need(ticket, "ticket");
+        *ticket = server->server->submit(id, x, k, y);
+    });
+}
+
+qh_status qh_server_wait(qh_server* server, uint64_t ticket, double timeout_s, int32_t* state) {
+    return guarded([&] {
+        need(server, "server");
+        need(state, "state");
+        *state = server->server->wait(ticket, timeout_s);
+    });
+}
+
+qh_status qh_server_get_stats(qh_server* server, qh_server_stats* out) {
+    return guarded([&] {
+        need(server, "server");
+        need(out, "stats");
+        const qhamd::ServerStats st = server->server->stats();
+        out->completed = st.completed;
+        out->completed_rhs = st.completed_rhs;
+        out->passes = st.passes;
+        out->resident_bytes = st.resident_bytes;
+        out->buffer_bytes = st.buffer_bytes;
+        out->p50_ms = st.p50_ms;
+        out->p99_ms = st.p99_ms;
+        out->mean_ms = st.mean_ms;
+    });
+}
+
+qh_status qh_server_reset_stats(qh_server* server) {
+    return guarded([&] {
+        need(server, "server");
+        server->server->reset_stats();
+    });
+}
+
+qh_status qh_server_closed_loop(qh_server* server, const int32_t* ids, uint64_t nids,
+                                const int64_t* ks, uint64_t nks, double rate, double duration,
+                                int32_t session, uint64_t seed, qh_load_result* out) {
+    return guarded([&] {
+        need(server, "server");
+        need(ids, "store ids");
+        need(ks, "widths");
+        need(out, "result");
+        const std::vector<int> id(ids, ids + nids);
+        const std::vector<std::int64_t> k(ks, ks + nks);
+        const qhamd::LoadResult r = server->server->closed_loop(id, k, rate, duration, session, seed);
+        out->offered_rate = r.offered_rate;
+        out->offered_rate_drawn = r.offered_rate_drawn;
+        out->sessions = r.sessions;
+        out->requests = r.requests;
+        out->passes = r.passes;
+        out->errors = r.errors;
+        out->seconds = r.seconds;
+        out->throughput_rps = r.throughput_rps;
+        out->p50_ms = r.p50_ms;
+        out->p99_ms = r.p99_ms;
+        out->mean_ms = r.mean_ms;
+    });
 }
 
 }  // extern "C"

@@ -2250,7 +2250,8 @@ void emulate_wide(const Plan& pl, const double* x, std::int64_t k, double* y, bo
 void destroy_device_store(DeviceStore* s) {
     if (!s) return;
     if (s->pins.load() > 0)
-        throw std::invalid_argument("store is in use by a batch: destroy the batch first");
+        throw std::invalid_argument(
+            "store is in use by a batch, factor plan or server: release that first");
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(s->device);

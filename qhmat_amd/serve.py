@@ -32,6 +32,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .device import apply as device_apply
+from .device import precision_code
 from .device import workspace_bytes
 
 
@@ -424,3 +425,177 @@ def closed_loop_load(server: Server, names: list[str], orders: dict[str, int], r
            "throughput_rps": done / elapsed, "passes": server.passes}
     out.update({f"latency_{k}_ms": v for k, v in server.latency_ms().items()})
     return out
+
+
+# ---- the library's serving loop (qh_server_*) ------------------------------------------
+
+
+def _server_sigs():
+    import ctypes as C
+
+    from . import _native as N
+
+    if getattr(_server_sigs, "done", False):
+        return N
+    _p, _u64, _i64, _i32, _dbl = C.c_void_p, C.c_uint64, C.c_int64, C.c_int32, C.c_double
+    N._sig("qh_server_create", C.POINTER(ServerConfig), C.POINTER(_p))
+    N._sig("qh_server_destroy", _p)
+    N._sig("qh_server_admit", _p, _p, C.POINTER(_i32))
+    N._sig("qh_server_evict", _p, _i32)
+    N._sig("qh_server_submit", _p, _i32, _p, _i64, _p, C.POINTER(_u64))
+    N._sig("qh_server_wait", _p, _u64, _dbl, C.POINTER(_i32))
+    N._sig("qh_server_get_stats", _p, C.POINTER(ServerStats))
+    N._sig("qh_server_reset_stats", _p)
+    N._sig("qh_server_closed_loop", _p, C.POINTER(_i32), _u64, C.POINTER(_i64), _u64, _dbl, _dbl,
+           _i32, _u64, C.POINTER(LoadResult))
+    _server_sigs.done = True
+    return N
+
+
+import ctypes as _C  # noqa: E402
+
+
+class ServerConfig(_C.Structure):
+    _fields_ = [("device", _C.c_int32), ("precision", _C.c_int32), ("slots", _C.c_int32),
+                ("max_rhs", _C.c_int64), ("hbm_budget", _C.c_uint64)]
+
+
+class ServerStats(_C.Structure):
+    _fields_ = [("completed", _C.c_uint64), ("completed_rhs", _C.c_uint64), ("passes", _C.c_uint64),
+                ("resident_bytes", _C.c_uint64), ("buffer_bytes", _C.c_uint64),
+                ("p50_ms", _C.c_double), ("p99_ms", _C.c_double), ("mean_ms", _C.c_double)]
+
+
+class LoadResult(_C.Structure):
+    _fields_ = [("offered_rate", _C.c_double), ("offered_rate_drawn", _C.c_double),
+                ("sessions", _C.c_uint64), ("requests", _C.c_uint64), ("passes", _C.c_uint64),
+                ("errors", _C.c_uint64), ("seconds", _C.c_double), ("throughput_rps", _C.c_double),
+                ("p50_ms", _C.c_double), ("p99_ms", _C.c_double), ("mean_ms", _C.c_double)]
+
+
+_TICKET_PENDING, _TICKET_DONE, _TICKET_FAILED = 0, 1, 2
+
+
+@dataclass
+class NativeRequest:
+    """A request queued on a NativeServer: wait() for its y."""
+
+    ticket: int
+    x: object
+    y: object
+
+
+class NativeServer:
+    """The serving loop of the library (qh_server_*): the same admission and
+    continuous batching as Server, with the dispatcher and the completion
+    thread native, so no interpreter lock is on a request's path.
+
+    Requests name a store admitted under a name; x is an FP64 device tensor
+    (n, k) whose values are complete when submitted (the server's passes run
+    on its own streams; submit synchronises the current stream when asked
+    to).  A pass merges every queued request of the store at the head of the
+    queue up to max_rhs columns (gathered into one apply and scattered back);
+    a request's result is bitwise qh_apply's for it alone on the same
+    arithmetic path, within the precision's tolerance otherwise.
+    """
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, precision: str = "fp64",
+                 slots: int = 3, max_rhs: int = 64):
+        N = _server_sigs()
+        self._N = N
+        self.device = device
+        self.precision = precision
+        self.max_rhs = int(max_rhs)
+        cfg = ServerConfig(device, precision_code(precision), slots, self.max_rhs, int(hbm_budget))
+        handle = _C.c_void_p()
+        N.check(N.lib().qh_server_create(_C.byref(cfg), _C.byref(handle)))
+        self._h = handle
+        self._ids: dict[str, int] = {}
+        self._natives: dict[str, object] = {}
+
+    def close(self) -> None:
+        """Stops the server; the admitted stores are released from it."""
+        if self._h:
+            self._N.check(self._N.lib().qh_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, name: str, native) -> None:
+        if name in self._ids:
+            raise ValueError(f"store {name!r} is already resident")
+        store = native.device_store(self.device)
+        sid = _C.c_int32()
+        self._N.check(self._N.lib().qh_server_admit(self._h, store, _C.byref(sid)))
+        self._ids[name] = int(sid.value)
+        self._natives[name] = native
+
+    def evict(self, name: str) -> None:
+        sid = self._ids.pop(name)
+        self._natives.pop(name)
+        self._N.check(self._N.lib().qh_server_evict(self._h, sid))
+
+    def submit(self, name: str, x, sync: bool = True) -> NativeRequest:
+        """Queues y = H_name x for an FP64 device tensor x (n,) or (n, k)."""
+        torch = __import__("torch")
+        native = self._natives[name]
+        x = x.reshape(native.order, -1)
+        if x.dtype != torch.float64 or not x.is_cuda or not x.is_contiguous():
+            raise ValueError("x must be a contiguous FP64 device tensor")
+        y = torch.empty_like(x)
+        if sync:
+            torch.cuda.current_stream(x.device).synchronize()
+        ticket = _C.c_uint64()
+        self._N.check(self._N.lib().qh_server_submit(self._h, self._ids[name], x.data_ptr(),
+                                                     x.shape[1], y.data_ptr(), _C.byref(ticket)))
+        return NativeRequest(int(ticket.value), x, y)
+
+    def wait(self, req: NativeRequest, timeout: float | None = None):
+        state = _C.c_int32()
+        self._N.check(self._N.lib().qh_server_wait(self._h, req.ticket,
+                                                   -1.0 if timeout is None else float(timeout),
+                                                   _C.byref(state)))
+        if state.value == _TICKET_PENDING:
+            raise TimeoutError("the request is not done")
+        if state.value == _TICKET_FAILED:
+            raise RuntimeError("the request failed (its store was evicted or its pass failed)")
+        return req.y
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._N.check(self._N.lib().qh_server_get_stats(self._h, _C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def reset_stats(self) -> None:
+        self._N.check(self._N.lib().qh_server_reset_stats(self._h))
+
+    def closed_loop(self, names: list[str], ks=(1, 4, 16), rate: float = 2000.0,
+                    duration: float = 3.0, session: int = 50, seed: int = 0) -> dict:
+        """Closed-loop GMRES-like sessions driven from the library's own
+        threads (qh_server_closed_loop); the figures closed_loop_load gives
+        for Server."""
+        ids = (_C.c_int32 * len(names))(*[self._ids[n] for n in names])
+        kk = (_C.c_int64 * len(ks))(*[int(k) for k in ks])
+        out = LoadResult()
+        self._N.check(self._N.lib().qh_server_closed_loop(self._h, ids, len(names), kk, len(ks),
+                                                          float(rate), float(duration),
+                                                          int(session), int(seed),
+                                                          _C.byref(out)))
+        r = {name: getattr(out, name) for name, _ in out._fields_}
+        return {"mode": "closed-loop sessions, native server", "offered_rate": r["offered_rate"],
+                "sessions": r["sessions"], "session_requests": session,
+                "offered_rate_drawn": r["offered_rate_drawn"], "requests": r["requests"],
+                "seconds": r["seconds"], "throughput_rps": r["throughput_rps"],
+                "passes": r["passes"], "errors": r["errors"], "latency_p50_ms": r["p50_ms"],
+                "latency_p99_ms": r["p99_ms"], "latency_mean_ms": r["mean_ms"]}
