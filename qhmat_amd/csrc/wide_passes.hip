@@ -522,21 +522,14 @@ __device__ __forceinline__ void left_item(const WideView& V, const ItemDev& f,
             }
         }
     };
-    // chunk 0 came with the fetch; chunk 1 is requested before chunk 0 is
-    // computed, later ones (more than 32 virtual rows: rare) where they are
-    // used
+    // chunk 0 came with the fetch; later chunks are requested where they
+    // are used
     const std::uint32_t nchunks = (nv + 15u) / 16u;
-    if (nchunks > 1) {
+    run_chunk(0, F.b);
+    for (std::uint32_t c = 1; c < nchunks; ++c) {
         Real b1[4][NS];
-        left_b<KC, FP32>(V, f, 1, q, r, p0, b1);
-        run_chunk(0, F.b);
-        run_chunk(1, b1);
-        for (std::uint32_t c = 2; c < nchunks; ++c) {
-            left_b<KC, FP32>(V, f, c, q, r, p0, b1);
-            run_chunk(c, b1);
-        }
-    } else {
-        run_chunk(0, F.b);
+        left_b<KC, FP32>(V, f, c, q, r, p0, b1);
+        run_chunk(c, b1);
     }
 #pragma unroll
     for (int i = 0; i < RB; ++i)
