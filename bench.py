@@ -61,6 +61,22 @@ def measured_traffic(kernel: str):
     return None
 
 
+TRACE_PROFILE = ROOT / "profiles" / "apply_square_exp_16384_kernel_trace.md"
+
+
+def traced_kernel_ms(kernel: str):
+    """Average execution time (ms) of `kernel` from the committed rocprofv3
+    kernel-trace summary of this workload (no dispatch gaps), or None."""
+    try:
+        lines = TRACE_PROFILE.read_text().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        if line.startswith("trace ") and line[len("trace "):].rsplit(" ", 1)[0] == kernel:
+            return float(line.rsplit(" ", 1)[1]) / 1e3
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
@@ -543,6 +559,15 @@ def main() -> int:
                      "kernel": dom_kernel,
                      "kernel_ms": round(dom_ms, 5), "kernel_algorithmic_bytes": dom_bytes},
     }
+    # the same kernel's execution time by the committed kernel trace (events
+    # include the dispatch gap before the launch, so the event figure errs low)
+    trace_ms = traced_kernel_ms(dom_kernel)
+    if trace_ms:
+        gbs = dom_bytes / (trace_ms * 1e-3) / 1e9
+        line["roofline"].update({"trace_kernel_ms": round(trace_ms, 5),
+                                 "achieved_trace": round(gbs, 2),
+                                 "frac_trace": round(gbs / HBM_PEAK_GBS, 4),
+                                 "trace_source": TRACE_PROFILE.name})
     for leg in legs.values():
         entries = {}
         for (k, prec) in sorted(leg.results):
