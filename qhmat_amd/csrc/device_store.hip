@@ -93,6 +93,22 @@ __device__ __forceinline__ Item item_at(const quad_t& r0, const quad_t& r1, std:
     return f;
 }
 
+// The fields of a record held in scalar registers (a wave's head record).
+__device__ __forceinline__ Item item_from(const ItemDev& d) {
+    Item f;
+    f.quad = d.quad;
+    f.quads = min(std::uint32_t(d.quads), std::uint32_t(kItemQuads));
+    f.kind = d.kind;
+    f.count = min(std::uint32_t(d.count), std::uint32_t(kItemComponents));
+    f.comp0 = d.comp0;
+    f.col = d.col;
+    f.cols = min(d.cols, std::uint32_t(kTile));
+    f.zrow = d.zrow;
+    f.zstride = d.zstride;
+    f.nseg = max(d.nseg, 1u);
+    return f;
+}
+
 // Columns of x an item reads (lanes = columns).
 __device__ __forceinline__ std::uint32_t x_columns(const Item& f) {
     return f.kind == kDense ? min(f.count, std::uint32_t(kDenseColumns))
@@ -558,16 +574,27 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
     valu_item<KB, FP32, PASS>(M, f, st, lane, p0, kk, R, wave_lds, acc);
 }
 
-// One wave's items: [b1, e1) followed by [b2, e2).  Row contributions
-// (kDense, kFused, kLeft) are added to acc (lanes = the R rows of the tile);
-// kRight items write their partial sums to the workspace.  Two items are in
-// flight while one is decoded: the fetch buffers alternate.
+// One wave's items: [b1, e1) followed by [b2, e2), as its head record gives
+// them.  Row contributions (kDense, kFused, kLeft) are added to acc (lanes =
+// the R rows of the tile); kRight items write their partial sums to the
+// workspace.  Two items are in flight while one is decoded: the fetch
+// buffers alternate.  The first two items' records come with the head, so
+// their fetches wait for one load, not for the run's records.
 template <int KB, bool FP32, int PASS>
-__device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1, std::uint32_t e1,
-                                          std::uint32_t b2, std::uint32_t e2, int p0, int kk,
-                                          std::uint32_t R, std::uint8_t* __restrict__ wave_lds,
-                                          double (&acc)[KB]) {
+__device__ __forceinline__ void run_items(const MemberView& M, const WaveHead* head, int p0,
+                                          int kk, std::uint32_t R,
+                                          std::uint8_t* __restrict__ wave_lds, double (&acc)[KB]) {
     const int lane = threadIdx.x & 63;
+    // the head as wave-uniform dwords (scalar registers whatever the
+    // compiler can prove about the pointer)
+    WaveHead h;
+    {
+        const std::uint32_t* src = reinterpret_cast<const std::uint32_t*>(head);
+        std::uint32_t* dst = reinterpret_cast<std::uint32_t*>(&h);
+#pragma unroll
+        for (int i = 0; i < int(sizeof(WaveHead) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    }
+    const std::uint32_t b1 = h.b1, e1 = h.e1, b2 = h.b2, e2 = h.e2;
     const std::uint32_t n1 = e1 - b1, total = n1 + (e2 - b2);
     for (std::uint32_t g0 = 0; g0 < total; g0 += 64) {
         const std::uint32_t n = min(64u, total - g0);
@@ -584,10 +611,14 @@ __device__ __forceinline__ void run_items(const MemberView& M, std::uint32_t b1,
         // the loads in flight are counted exactly and never waited for early.
         const Item none = {};
         auto slot = [&](std::uint32_t i) { return i < n ? item_at(r0, r1, i) : none; };
-        Item fa = slot(0), fb = slot(1), fc = slot(2), fd = slot(3);
+        Item fa = g0 == 0 ? item_from(h.first[0]) : slot(0);
+        Item fb = g0 == 0 ? item_from(h.first[1]) : slot(1);
         Fetch<KB> FA, FB;
         fetch<KB>(M, fa, lane, p0, kk, FA);
         fetch<KB>(M, fb, lane, p0, kk, FB);
+        // the records of items 2 and 3 are read only now, so that the first
+        // two fetches do not wait for the group's record loads
+        Item fc = slot(2), fd = slot(3);
 
         // Steady state: buffer A holds item s, buffer B item s+1.  Step s
         // waits for item s (fetched two steps earlier) and fetches item s+2
@@ -680,8 +711,8 @@ __device__ __forceinline__ void pass1_rows(const MemberView& M, std::uint32_t gr
     double total[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) total[p] = 0.0;
-    run_items<KB, FP32, 1>(M, tile->rsplit[r], tile->rsplit[r + 1], tile->split1[r],
-                           tile->split1[r + 1], p0, kk, R, lds + wave * wave_lds_bytes<KB>(), total);
+    run_items<KB, FP32, 1>(M, M.heads + t * kWaves + std::uint32_t(r), p0, kk, R,
+                           lds + wave * wave_lds_bytes<KB>(), total);
     double zero[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) zero[p] = 0.0;
@@ -725,8 +756,8 @@ __device__ __forceinline__ void tile_pass2(const MemberView& M, std::uint32_t t,
     double total[KB];
 #pragma unroll
     for (int p = 0; p < KB; ++p) total[p] = 0.0;
-    run_items<KB, FP32, 2>(M, 0u, 0u, tile->split2[wave], tile->split2[wave + 1], p0, kk, R,
-                           lds + wave * wave_lds_bytes<KB>(), total);
+    run_items<KB, FP32, 2>(M, M.heads + (M.tiles_count + t) * kWaves + std::uint32_t(wave), p0, kk,
+                           R, lds + wave * wave_lds_bytes<KB>(), total);
     __syncthreads();  // every wave is done with its stage
     double* part = reinterpret_cast<double*>(lds);  // [wave][p][row]
 #pragma unroll
@@ -980,6 +1011,7 @@ MemberView member_view(const DeviceStore& s, const double* x, std::int64_t ldx, 
     M.cscale = s.cscale;
     M.items = s.items;
     M.tiles = s.tiles;
+    M.heads = s.heads;
     M.x = x;
     M.y = y;
     M.zp = zp;
@@ -1701,6 +1733,24 @@ Plan build_plan(const qh_store_desc& d) {
         T.rsplit[w + 1] = next;
     }
     plan.z_rows = z_rows;
+    // each wave's run and its first two records
+    plan.heads.assign(2 * ntiles * std::size_t(kWaves), WaveHead{});
+    for (std::size_t t = 0; t < ntiles; ++t) {
+        const TileDev& T = plan.tiles[t];
+        for (int w = 0; w < kWaves; ++w) {
+            for (int pass = 0; pass < 2; ++pass) {
+                WaveHead& h = plan.heads[(pass * ntiles + t) * kWaves + std::size_t(w)];
+                if (pass == 0) {
+                    h.b1 = T.rsplit[w], h.e1 = T.rsplit[w + 1], h.b2 = T.split1[w], h.e2 = T.split1[w + 1];
+                } else {
+                    h.b1 = h.e1 = 0, h.b2 = T.split2[w], h.e2 = T.split2[w + 1];
+                }
+                const std::uint32_t n1 = h.e1 - h.b1, total = n1 + (h.e2 - h.b2);
+                for (std::uint32_t i = 0; i < 2 && i < total; ++i)
+                    h.first[i] = items[i < n1 ? h.b1 + i : h.b2 + (i - n1)];
+            }
+        }
+    }
     build_wide(plan, right_items, pass1, pass2, row_tiles, col_tiles);
 
     qh_store_stats& st = plan.stats;
@@ -1839,6 +1889,7 @@ DeviceStore* create_device_store(const qh_store_desc& d, int device) {
         s->cscale = upload(plan.cscale, bytes);
         s->items = upload(plan.items, bytes);
         s->tiles = upload(plan.tiles, bytes);
+        s->heads = upload(plan.heads, bytes);
         s->witems = upload(plan.witems, bytes);
         s->wruns_r = upload(plan.wruns_r, bytes);
         s->wruns_l = upload(plan.wruns_l, bytes);
@@ -1907,6 +1958,24 @@ void emulate_plan(const Plan& pl, const double* x, std::int64_t k, double* y) {
         const std::uint64_t sign = std::uint64_t{1} << (bits - 1);
         return double(std::int64_t(raw ^ sign) - std::int64_t(sign));
     };
+    // every wave's head record: its run bounds and first two records
+    check(pl.heads.size() == 2 * pl.tiles.size() * std::size_t(kWaves), "wave head count");
+    for (std::size_t t = 0; t < pl.tiles.size(); ++t)
+        for (int w = 0; w < kWaves; ++w)
+            for (int pass = 0; pass < 2; ++pass) {
+                const TileDev& T = pl.tiles[t];
+                const WaveHead& h = pl.heads[(pass * pl.tiles.size() + t) * kWaves + std::size_t(w)];
+                const bool bounds =
+                    pass == 0 ? h.b1 == T.rsplit[w] && h.e1 == T.rsplit[w + 1] && h.b2 == T.split1[w] &&
+                                    h.e2 == T.split1[w + 1]
+                              : h.b1 == 0 && h.e1 == 0 && h.b2 == T.split2[w] && h.e2 == T.split2[w + 1];
+                check(bounds, "wave head bounds");
+                const std::uint32_t n1 = h.e1 - h.b1, total = n1 + (h.e2 - h.b2);
+                for (std::uint32_t i = 0; i < 2; ++i) {
+                    const ItemDev want = i < total ? at(pl.items, i < n1 ? h.b1 + i : h.b2 + (i - n1)) : ItemDev{};
+                    check(std::memcmp(&want, &h.first[i], sizeof(ItemDev)) == 0, "wave head record");
+                }
+            }
     auto check_item = [&](const ItemDev& it) {
         check(it.quads <= kItemQuads, "item size");
         check(it.count >= 1 && it.count <= kItemComponents, "item component count");
@@ -2259,6 +2328,7 @@ void destroy_device_store(DeviceStore* s) {
     for (void* p : {static_cast<void*>(s->perm), static_cast<void*>(s->words),
                     static_cast<void*>(s->widths), static_cast<void*>(s->cscale),
                     static_cast<void*>(s->items), static_cast<void*>(s->tiles),
+                    static_cast<void*>(s->heads),
                     static_cast<void*>(s->witems), static_cast<void*>(s->wruns_r),
                     static_cast<void*>(s->wruns_l), static_cast<void*>(s->wsums),
                     static_cast<void*>(s->wzsums),

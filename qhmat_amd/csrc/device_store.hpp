@@ -74,6 +74,19 @@ struct ItemDev {
 };
 static_assert(sizeof(ItemDev) == 32, "ItemDev is 32 bytes");
 
+// Where a narrow-pass wave starts: its run [b1, e1) followed by [b2, e2)
+// (pass 1: a share of the kRight items, then the tile's own items; pass 2:
+// an empty first range, then the tile's kLeft items) and the records of the
+// run's first two items (zero records past its end), so that a wave's first
+// fetches need one memory round trip instead of three (bounds, records,
+// data).  Wave r of tile t in pass P (1 or 2) reads heads[((P - 1) tiles +
+// t) kWaves + r].
+struct WaveHead {
+    std::uint32_t b1, e1, b2, e2;
+    ItemDev first[2];
+};
+static_assert(sizeof(WaveHead) == 80, "WaveHead is 80 bytes");
+
 // Row tile: rows [row0, row0+rows).  Run w (of kWaves) of the tile's pass-1
 // workgroups streams the kRight items [rsplit[w], rsplit[w+1]) (a share of
 // all right factors, independent of the tile) and the tile's items
@@ -169,11 +182,12 @@ struct MemberView {
     const double* cscale;        // lscale * rscale per component
     const ItemDev* items;
     const TileDev* tiles;
+    const WaveHead* heads;
     const double* x;
     double* y;
-    double* zp;     // workspace: z_rows x k partial sums, x in clustered order after the
-                    // row sets; row sets: 2 kPass1Groups sets of
-                    // order x k row sums (four per pass)
+    double* zp;     // workspace: z_rows x k partial sums, then the row sets (kPass1Groups
+                    // sets of order x k row sums, one per pass-1 workgroup of a tile),
+                    // then x in clustered order
     std::int64_t ldx, ldy;
     std::int32_t k;
     std::uint32_t tiles_count;  // row tiles
@@ -221,6 +235,7 @@ struct DeviceStore {
     double* cscale = nullptr;
     ItemDev* items = nullptr;
     TileDev* tiles = nullptr;
+    WaveHead* heads = nullptr;
     // wide passes
     ItemDev* witems = nullptr;
     WideRun* wruns_r = nullptr;
@@ -276,6 +291,7 @@ struct Plan {
     std::vector<double> cscale;
     std::vector<ItemDev> items;
     std::vector<TileDev> tiles;
+    std::vector<WaveHead> heads;  // 2 x tiles x kWaves
     std::vector<std::int64_t> block_to_lowrank;
     std::vector<std::vector<ChunkRef>> left_chunks, right_chunks;
     std::vector<std::uint32_t> comp_base, ranks;
