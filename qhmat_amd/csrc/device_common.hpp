@@ -82,6 +82,18 @@ inline std::uint32_t xcd_grid(std::uint64_t groups, std::uint64_t slices) {
     return std::uint32_t((groups + kXcds - 1) / kXcds * kXcds * slices);
 }
 
+// Issue priority by progress (four levels): a wave that has done less of
+// its run than the waves it shares a SIMD with is issued first, so that the
+// waves of a SIMD finish together (by default the oldest wave is issued
+// first and the youngest finishes last, running alone).  done and total are
+// wave-uniform.
+__device__ __forceinline__ void progress_priority(std::uint32_t done, std::uint32_t total) {
+    if (4 * done < total) __builtin_amdgcn_s_setprio(3);
+    else if (4 * done < 2 * total) __builtin_amdgcn_s_setprio(2);
+    else if (4 * done < 3 * total) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 // LDS written by some lanes of a wave and read by others: keep the compiler
 // from moving accesses across this point and wait for the LDS queue.
 __device__ __forceinline__ void wave_lds_sync() {

@@ -579,7 +579,8 @@ __device__ __forceinline__ void compute_item(const MemberView& M, const Item& f,
 // the R rows of the tile); kRight items write their partial sums to the
 // workspace.  Two items are in flight while one is decoded: the fetch
 // buffers alternate.  The first two items' records come with the head, so
-// their fetches wait for one load, not for the run's records.
+// their fetches wait for one load, not for the run's records.  Each step
+// sets the wave's issue priority by its progress through the run.
 template <int KB, bool FP32, int PASS>
 __device__ __forceinline__ void run_items(const MemberView& M, const WaveHead* head, int p0,
                                           int kk, std::uint32_t R,
@@ -626,6 +627,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, const WaveHead* h
         // decode.
         for (std::uint32_t s = 0; s < n; s += 2) {
             {
+                progress_priority(g0 + s, total);
                 stage_item<KB, FP32>(fa, FA, lane, wave_lds);
                 fetch<KB>(M, fc, lane, p0, kk, FA);
                 wave_lds_sync();
@@ -639,6 +641,7 @@ __device__ __forceinline__ void run_items(const MemberView& M, const WaveHead* h
             // read zero, so that both steps always run and the waits for
             // loads in flight stay exact across the loop.
             {
+                progress_priority(g0 + s + 1, total);
                 stage_item<KB, FP32>(fb, FB, lane, wave_lds);
                 fetch<KB>(M, fd, lane, p0, kk, FB);
                 wave_lds_sync();
@@ -1189,21 +1192,42 @@ std::size_t first_tile(const std::vector<std::pair<std::uint64_t, std::uint64_t>
            tiles.begin();
 }
 
-// Cuts `cost` into `parts` contiguous runs of similar total; out[i] is the
-// first index of run i (out has parts + 1 entries, offset by `base`).
+// Cuts `cost` into `parts` contiguous runs whose totals follow `weight`
+// (equal when null); out[i] is the first index of run i (out has parts + 1
+// entries, offset by `base`).
 void split_runs(const std::vector<std::uint64_t>& cost, std::uint32_t parts, std::uint32_t base,
-                std::uint32_t* out) {
+                std::uint32_t* out, const double* weight = nullptr) {
     std::uint64_t total = 0;
     for (auto c : cost) total += c;
+    auto weight_of = [&](std::uint32_t w) { return weight ? weight[w] : 1.0; };
+    double wsum = 0.0;
+    for (std::uint32_t w = 0; w < parts; ++w) wsum += weight_of(w);
     std::uint32_t e = 0;
     std::uint64_t run = 0;
+    double wdone = 0.0;
     out[0] = base;
     for (std::uint32_t w = 1; w < parts; ++w) {
-        const std::uint64_t goal = total * w / parts;
+        wdone += weight_of(w - 1);
+        const std::uint64_t goal = std::uint64_t(double(total) * wdone / wsum);
         while (e < cost.size() && run + cost[e] / 2 < goal) run += cost[e++];
         out[w] = base + e;
     }
     out[parts] = base + std::uint32_t(cost.size());
+}
+
+// Among the waves that share a SIMD, issue goes to the oldest first (at
+// equal priority), so the wave dispatched first runs fastest and the last
+// slowest: with equal work, the pass-1 waves of the four workgroups a CU
+// holds took 15, 17, 20 and 22 us (square_exp n=16384, k=1).  The static
+// splits therefore give a wave a share of the work by its dispatch rank on
+// its SIMD: pass-1 workgroup g has rank (g / kDeviceCUs) mod 4 (workgroups
+// go round the CUs in launch order), pass-2 wave w of a tile's workgroup
+// rank w / 4.  The weights are 1 / those times, normalised; the waves'
+// progress priority (progress_priority) evens out the rest at run time.
+constexpr std::uint32_t kDeviceCUs = 256;
+double age_weight(std::uint32_t rank) {
+    constexpr double kWeight[4] = {1.24, 1.09, 0.93, 0.85};
+    return kWeight[rank % 4];
 }
 
 // Components of an item wider than 24 bits (split in FP32 mode).
@@ -1697,7 +1721,9 @@ Plan build_plan(const qh_store_desc& d) {
             std::vector<std::uint64_t> cost;
             for (const auto& it : list) cost.push_back(cost_of(it));
             std::uint32_t* split = pass == 0 ? T.split1 : T.split2;
-            split_runs(cost, kWaves, std::uint32_t(items.size()), split);
+            double weight[kWaves];
+            for (int w = 0; w < kWaves; ++w) weight[w] = pass == 0 ? 1.0 : age_weight(std::uint32_t(w) / 4);
+            split_runs(cost, kWaves, std::uint32_t(items.size()), split, weight);
             if (pass == 0)
                 for (int w = 0; w < kWaves; ++w) {
                     std::uint64_t c = 0;
@@ -1709,21 +1735,30 @@ Plan build_plan(const qh_store_desc& d) {
         }
     }
     require(items.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
-    // The right-factor items fill the pass-1 runs up to an even total: run r
-    // takes items (in order) while the cost of runs 0..r stays under r + 1
-    // means, so that rounding does not pile up in the last run.
+    // The right-factor items fill the pass-1 runs up to totals that follow
+    // the runs' dispatch-rank weights: run r takes items (in order) while
+    // the cost of runs 0..r stays under their share of the whole, so that
+    // rounding does not pile up in the last run.
     std::uint64_t all = 0;
     for (auto c : run_cost) all += c;
     all += right_load;
-    const std::uint64_t nruns = std::max<std::uint64_t>(1, run_cost.size());
+    std::vector<double> run_weight(run_cost.size());
+    double wsum = 0.0;
+    for (std::size_t r = 0; r < run_cost.size(); ++r) {
+        const std::uint32_t group = std::uint32_t(r / kGroupWaves);
+        run_weight[r] = age_weight(group / kDeviceCUs);
+        wsum += run_weight[r];
+    }
     std::uint32_t next = 0;
     std::uint64_t done = 0;  // cost of the runs filled so far
+    double wdone = 0.0;
     for (std::size_t r = 0; r < run_cost.size(); ++r) {
         TileDev& T = plan.tiles[r / kWaves];
         const int w = int(r % kWaves);
         T.rsplit[w] = next;
         done += run_cost[r];
-        const std::uint64_t goal = all * (r + 1) / nruns;
+        wdone += run_weight[r];
+        const std::uint64_t goal = std::uint64_t(double(all) * wdone / wsum);
         const bool last = r + 1 == run_cost.size();
         while (next < right_items.size() &&
                (last || done + cost_of(right_items[next]) / 2 <= goal)) {
