@@ -1352,14 +1352,38 @@ void build_wide(Plan& plan, const std::vector<ItemDev>& right_items,
             rc[c].push_back(wide_right_cost(it, widths));
             rtotal += rc[c].back();
         }
-    const std::uint64_t runit = std::max<std::uint64_t>(1, rtotal / std::max<std::uint64_t>(kTargetGroups, nct));
+    // Workgroups per column tile.  All of pass R's workgroups fit on the
+    // device at once (kTargetGroups, four per CU), and a workgroup's share
+    // follows the dispatch rank on its CU that its size gives it once the
+    // workgroups are ordered costliest first (below): the tiles, costliest
+    // first, take workgroups of age_weight(rank) units each.
+    std::vector<std::uint64_t> tile_cost(nct, 0);
+    std::vector<std::size_t> by_cost;
+    for (std::size_t c = 0; c < nct; ++c) {
+        for (auto v : rc[c]) tile_cost[c] += v;
+        if (!vitems[c].empty()) by_cost.push_back(c);
+    }
+    std::stable_sort(by_cost.begin(), by_cost.end(),
+                     [&](std::size_t a, std::size_t b) { return tile_cost[a] > tile_cost[b]; });
+    std::vector<std::uint32_t> tile_groups(nct, 0);
+    double wsum = 0.0;
+    for (std::uint32_t g = 0; g < kTargetGroups; ++g) wsum += age_weight(g / kDeviceCUs);
+    double runit = std::max(1.0, double(rtotal) / std::max(wsum, double(nct)));
+    for (int attempt = 0; attempt < 32; ++attempt) {
+        std::uint64_t placed = 0;
+        for (std::size_t c : by_cost) {
+            const double unit = runit * age_weight(std::uint32_t(placed / kDeviceCUs));
+            const std::uint64_t cap = std::min<std::uint64_t>(64, std::max<std::size_t>(1, (vitems[c].size() + 1) / 2));
+            tile_groups[c] = std::uint32_t(std::clamp<std::uint64_t>(
+                std::uint64_t(double(tile_cost[c]) / unit + 0.5), 1, cap));
+            placed += tile_groups[c];
+        }
+        if (placed <= std::max<std::uint64_t>(kTargetGroups, by_cost.size())) break;
+        runit *= 1.02;
+    }
     for (std::size_t c = 0; c < nct; ++c) {
         if (vitems[c].empty()) continue;
-        std::uint64_t cost = 0;
-        for (auto v : rc[c]) cost += v;
-        const std::uint64_t cap = std::max<std::uint64_t>(1, (vitems[c].size() + 1) / 2);
-        const std::uint32_t groups =
-            std::uint32_t(std::clamp<std::uint64_t>((cost + runit / 2) / runit, 1, std::min<std::uint64_t>(cap, 64)));
+        const std::uint32_t groups = tile_groups[c];
         std::vector<std::uint32_t> cut(groups * kWideWaves + 1);
         split_runs(rc[c], groups * kWideWaves, std::uint32_t(out.size()), cut.data());
         out.insert(out.end(), vitems[c].begin(), vitems[c].end());
@@ -1397,6 +1421,33 @@ void build_wide(Plan& plan, const std::vector<ItemDev>& right_items,
         plan.wsets += runs;
     }
     require(out.size() < (std::uint64_t{1} << 31), "store too large for one device plan");
+    // Workgroups in launch order costliest first (by their costliest wave):
+    // the oldest workgroups on a CU are issued first and run fastest, and
+    // the last ones dispatched, which start as others finish and set the
+    // pass's tail, are the shortest.
+    auto costliest_first = [&](std::vector<WideRun>& runs, auto item_cost) {
+        const std::size_t groups = runs.size() / kWideWaves;
+        std::vector<std::uint64_t> cost(groups, 0);
+        for (std::size_t g = 0; g < groups; ++g)
+            for (int w = 0; w < kWideWaves; ++w) {
+                const WideRun& r = runs[g * kWideWaves + w];
+                std::uint64_t c = 0;
+                for (std::uint32_t i = r.begin; i < r.end; ++i) c += item_cost(out[i], widths);
+                cost[g] = std::max(cost[g], c);
+            }
+        std::vector<std::size_t> order(groups);
+        for (std::size_t g = 0; g < groups; ++g) order[g] = g;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](std::size_t a, std::size_t b) { return cost[a] > cost[b]; });
+        std::vector<WideRun> sorted;
+        sorted.reserve(runs.size());
+        for (std::size_t g : order)
+            sorted.insert(sorted.end(), runs.begin() + g * kWideWaves,
+                          runs.begin() + (g + 1) * kWideWaves);
+        runs = std::move(sorted);
+    };
+    costliest_first(plan.wruns_r, wide_right_cost);
+    costliest_first(plan.wruns_l, wide_left_cost);
 }
 
 }  // namespace
