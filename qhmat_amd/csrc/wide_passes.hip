@@ -249,6 +249,9 @@ __device__ __forceinline__ void right_group(const WideView& V, std::uint32_t g, 
     RightFetch F;
     right_fetch(V, f, lane, F);
     for (std::uint32_t i = begin; i < end; ++i) {
+        // issue priority by progress in FP32-split mode (sphere k=64: 359 -> 349
+        // us per pass R); FP64 pass R was slower with it (506 -> 514 us)
+        if constexpr (FP32) progress_priority(i - begin, end - begin);
         // stage item i, then fetch item i + 1 while it is decoded
 #pragma unroll
         for (int t = 0; t < kItemQuads / 64; ++t)
