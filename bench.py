@@ -447,8 +447,9 @@ def main() -> int:
     if 1 not in args.rhs:
         args.rhs = [1] + args.rhs  # the headline value is quoted at k=1
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the apply has no multi-GPU path (a store would "
+              "shard by ranges of row tiles, with x all-gathered before the right-factor pass)",
+              file=sys.stderr)
         return 2
 
     import numpy as np
